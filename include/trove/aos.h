@@ -1,0 +1,344 @@
+// Array-of-structures access: the low-level warp-contiguous path and the
+// dynamic per-lane-address path with its tile cascade.
+//
+// Public interface of trove/aos.h:66-105 and 258-290:
+//   load_warp_contiguous<tile>(p) / store_warp_contiguous<tile>(v, p):
+//     every lane of a converged tile passes p = tile base + its rank. The tile
+//     moves its tile*sizeof(T) bytes with m coalesced word accesses per lane
+//     (word j of lane l is tile word l + j*tile), then R2C / C2R puts struct
+//     l in lane l.
+//   load(p) / store(v, p): arbitrary per-lane addresses. The widest converged
+//     tile (64, 32, ..., 2 lanes) is used; lanes in no converged pair fall
+//     back to per-lane word accesses.
+// Eligibility is the reference's (aos.h:40-61): sizeof(T) % 4 == 0 and
+// 1 < m <= tile; everything else is a per-lane access.
+//
+// Dynamic path, per tile (aos.h:110-219 for the reference's version):
+//   1. p0 = the address of the tile's rank-0 lane (readfirstlane for a whole
+//      wavefront, one 64-bit bpermute for a smaller tile).
+//   2. If every lane holds p0 + rank, the access is warp-contiguous and takes
+//      the coalesced path (the reference has this check commented out,
+//      aos.h:205-207, 234-236).
+//   3. Otherwise word j of lane l belongs to tile struct s = (l + j*tile) / m
+//      at word (l + j*tile) % m; its address comes from lane s. When every
+//      lane's offset from p0 fits in 32 bits (the usual case: one array) the
+//      offset is broadcast with one ds_bpermute instead of a 64-bit pointer
+//      (two).
+#pragma once
+#include <trove/array.h>
+#include <trove/memory.h>
+#include <trove/transpose.h>
+#include <trove/utility.h>
+#include <trove/warp.h>
+#include <trove/detail/dismember.h>
+#include <trove/detail/fallback.h>
+
+namespace trove {
+
+namespace detail {
+
+template <typename T, std::size_t tile_size>
+struct size_in_range {
+    typedef typename dismember_type<T>::type U;
+    static const int size = aliased_size<T, U>::value;
+    static const bool value = (size > 1) && (size <= (int)tile_size);
+};
+
+template <typename T, std::size_t tile_size>
+struct use_shfl {
+    static const bool value = size_multiple_power_of_two<T, 2>::value && size_in_range<T, tile_size>::value;
+};
+
+template <typename T, std::size_t tile_size>
+struct use_direct {
+    static const bool value = !use_shfl<T, tile_size>::value;
+};
+
+template <typename T>
+struct word_layout {
+    typedef typename dismember_type<T>::type U;
+    static constexpr int m = aliased_size<T, U>::value;
+    static constexpr int W = sizeof(U) / 4;
+};
+
+// Coalesced loads of a tile's words: word j of rank id is base[id + j*n].
+template <int n, typename T>
+__device__ __forceinline__ void tile_load_words(const T* tile_base, int id,
+                                                uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
+    typedef word_layout<T> L;
+    const typename L::U* base = reinterpret_cast<const typename L::U*>(tile_base);
+#pragma unroll
+    for (int j = 0; j < L::m; ++j) {
+        const typename L::U u = base[id + j * n];
+        __builtin_memcpy(&v[j * L::W], &u, sizeof(u));
+    }
+}
+
+template <int n, typename T>
+__device__ __forceinline__ void tile_store_words(T* tile_base, int id,
+                                                 const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
+    typedef word_layout<T> L;
+    typename L::U* base = reinterpret_cast<typename L::U*>(tile_base);
+#pragma unroll
+    for (int j = 0; j < L::m; ++j) {
+        typename L::U u;
+        __builtin_memcpy(&u, &v[j * L::W], sizeof(u));
+        base[id + j * n] = u;
+    }
+}
+
+// Address of the tile's rank-0 lane, seen by every lane of the tile.
+template <int n, typename P>
+__device__ __forceinline__ P* tile_first_pointer(P* p, int first) {
+    if constexpr (n == 64) {
+        return broadcast_first(p);
+    } else {
+        return bpermute(p, first << 2);
+    }
+}
+
+// Word j of rank id in a gathered tile: struct s = (id + j*n) / m, word u.
+// Walks (s, u) incrementally over j.
+template <int n, int m>
+struct word_walk {
+    int s, u;
+    __device__ __forceinline__ explicit word_walk(int id) : s(id / m), u(id % m) {}
+    __device__ __forceinline__ void next() {
+        u += n % m;
+        if (u >= m) {
+            u -= m;
+            s += 1;
+        }
+        s += n / m;
+    }
+};
+
+// Loads the words of a tile whose structs sit at arbitrary per-lane
+// addresses, in the column-side layout (word j of rank id = tile word
+// id + j*n of the concatenated structs).
+template <int n, typename T>
+__device__ __forceinline__ void tile_gather_words(const T* src, const T* p0, long long delta, int id, int first,
+                                                  uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
+    typedef word_layout<T> L;
+    typedef typename L::U U;
+    word_walk<n, L::m> w(id);
+    if (tile_all<n>(delta == (long long)(int)delta)) {
+        const int d32 = (int)delta;
+#pragma unroll
+        for (int j = 0; j < L::m; ++j) {
+            const int ds = __builtin_amdgcn_ds_bpermute((first + w.s) << 2, d32);
+            const U u = reinterpret_cast<const U*>(p0 + ds)[w.u];
+            __builtin_memcpy(&v[j * L::W], &u, sizeof(u));
+            w.next();
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < L::m; ++j) {
+            const T* ps = bpermute(src, (first + w.s) << 2);
+            const U u = reinterpret_cast<const U*>(ps)[w.u];
+            __builtin_memcpy(&v[j * L::W], &u, sizeof(u));
+            w.next();
+        }
+    }
+}
+
+template <int n, typename T>
+__device__ __forceinline__ void tile_scatter_words(T* dst, T* p0, long long delta, int id, int first,
+                                                   const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
+    typedef word_layout<T> L;
+    typedef typename L::U U;
+    word_walk<n, L::m> w(id);
+    if (tile_all<n>(delta == (long long)(int)delta)) {
+        const int d32 = (int)delta;
+#pragma unroll
+        for (int j = 0; j < L::m; ++j) {
+            const int ds = __builtin_amdgcn_ds_bpermute((first + w.s) << 2, d32);
+            U u;
+            __builtin_memcpy(&u, &v[j * L::W], sizeof(u));
+            reinterpret_cast<U*>(p0 + ds)[w.u] = u;
+            w.next();
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < L::m; ++j) {
+            T* ps = bpermute(dst, (first + w.s) << 2);
+            U u;
+            __builtin_memcpy(&u, &v[j * L::W], sizeof(u));
+            reinterpret_cast<U*>(ps)[w.u] = u;
+            w.next();
+        }
+    }
+}
+
+// Tile gather from one array: rank id wants base[index]. Produces the
+// column-side layout of the gathered structs (what a coalesced tile store
+// expects), broadcasting the 32-bit index of each word's owner.
+template <int n, typename T>
+__device__ __forceinline__ void tile_gather_indexed(const T* base, int index, int id, int first,
+                                                    uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
+    typedef word_layout<T> L;
+    typedef typename L::U U;
+    word_walk<n, L::m> w(id);
+#pragma unroll
+    for (int j = 0; j < L::m; ++j) {
+        const int ds = __builtin_amdgcn_ds_bpermute((first + w.s) << 2, index);
+        const U u = reinterpret_cast<const U*>(base + ds)[w.u];
+        __builtin_memcpy(&v[j * L::W], &u, sizeof(u));
+        w.next();
+    }
+}
+
+// Tile scatter into one array: the structs held in column-side layout go to
+// base[index of their owner rank].
+template <int n, typename T>
+__device__ __forceinline__ void tile_scatter_indexed(T* base, int index, int id, int first,
+                                                     const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
+    typedef word_layout<T> L;
+    typedef typename L::U U;
+    word_walk<n, L::m> w(id);
+#pragma unroll
+    for (int j = 0; j < L::m; ++j) {
+        const int ds = __builtin_amdgcn_ds_bpermute((first + w.s) << 2, index);
+        U u;
+        __builtin_memcpy(&u, &v[j * L::W], sizeof(u));
+        reinterpret_cast<U*>(base + ds)[w.u] = u;
+        w.next();
+    }
+}
+
+}  // namespace detail
+
+template <std::size_t tile_size = WARP_SIZE, typename T>
+__device__ typename enable_if<detail::use_shfl<T, tile_size>::value, T>::type load_warp_contiguous(const T* src) {
+    constexpr int n = (int)tile_size;
+    typedef detail::word_layout<T> L;
+    const int lane = lane_id();
+    const int id = lane & (n - 1);
+    uint32_t v[L::m * L::W];
+    detail::tile_load_words<n>(src - id, id, v);
+    detail::r2c_words<n, L::m, L::W>(v, id, lane & ~(n - 1));
+    T out;
+    __builtin_memcpy(&out, v, sizeof(T));
+    return out;
+}
+
+template <std::size_t tile_size = WARP_SIZE, typename T>
+__device__ typename enable_if<detail::use_direct<T, tile_size>::value, T>::type load_warp_contiguous(const T* src) {
+    return detail::divergent_load(src);
+}
+
+template <std::size_t tile_size = WARP_SIZE, typename T>
+__device__ typename enable_if<detail::use_shfl<T, tile_size>::value>::type store_warp_contiguous(const T& data,
+                                                                                                 T* dest) {
+    constexpr int n = (int)tile_size;
+    typedef detail::word_layout<T> L;
+    const int lane = lane_id();
+    const int id = lane & (n - 1);
+    uint32_t v[L::m * L::W];
+    __builtin_memcpy(v, &data, sizeof(T));
+    detail::c2r_words<n, L::m, L::W>(v, id, lane & ~(n - 1));
+    detail::tile_store_words<n>(dest - id, id, v);
+}
+
+template <std::size_t tile_size = WARP_SIZE, typename T>
+__device__ typename enable_if<detail::use_direct<T, tile_size>::value>::type store_warp_contiguous(const T& data,
+                                                                                                   T* dest) {
+    detail::divergent_store(data, dest);
+}
+
+namespace detail {
+
+template <typename T, typename Tile>
+__device__ typename enable_if<use_shfl<T, Tile::size()>::value, T>::type load_dispatch(const T* src,
+                                                                                       const Tile& tile) {
+    constexpr int n = Tile::size();
+    typedef word_layout<T> L;
+    const int id = tile.id();
+    const int first = tile.first_lane();
+    const T* p0 = tile_first_pointer<n>(src, first);
+    const long long delta = (long long)(src - p0);
+    uint32_t v[L::m * L::W];
+    if (tile_all<n>(delta == id)) {
+        tile_load_words<n>(p0, id, v);
+    } else {
+        tile_gather_words<n>(src, p0, delta, id, first, v);
+    }
+    r2c_words<n, L::m, L::W>(v, id, first);
+    T out;
+    __builtin_memcpy(&out, v, sizeof(T));
+    return out;
+}
+
+template <typename T, typename Tile>
+__device__ typename enable_if<use_direct<T, Tile::size()>::value, T>::type load_dispatch(const T* src, const Tile&) {
+    return divergent_load(src);
+}
+
+template <typename T, typename Tile>
+__device__ typename enable_if<use_shfl<T, Tile::size()>::value>::type store_dispatch(const T& data, T* dest,
+                                                                                     const Tile& tile) {
+    constexpr int n = Tile::size();
+    typedef word_layout<T> L;
+    const int id = tile.id();
+    const int first = tile.first_lane();
+    T* p0 = tile_first_pointer<n>(dest, first);
+    const long long delta = (long long)(dest - p0);
+    uint32_t v[L::m * L::W];
+    __builtin_memcpy(v, &data, sizeof(T));
+    c2r_words<n, L::m, L::W>(v, id, first);
+    if (tile_all<n>(delta == id)) {
+        tile_store_words<n>(p0, id, v);
+    } else {
+        tile_scatter_words<n>(dest, p0, delta, id, first, v);
+    }
+}
+
+template <typename T, typename Tile>
+__device__ typename enable_if<use_direct<T, Tile::size()>::value>::type store_dispatch(const T& data, T* dest,
+                                                                                       const Tile&) {
+    divergent_store(data, dest);
+}
+
+}  // namespace detail
+
+// Load *src cooperatively with the widest converged tile around the lane.
+template <typename T>
+__device__ T load(const T* src) {
+    if (warp_converged<WARP_SIZE>()) {
+        return detail::load_dispatch(src, thread_tile<WARP_SIZE>());
+    } else if (warp_converged<WARP_SIZE / 2>()) {
+        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 2>());
+    } else if (warp_converged<WARP_SIZE / 4>()) {
+        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 4>());
+    } else if (warp_converged<WARP_SIZE / 8>()) {
+        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 8>());
+    } else if (warp_converged<WARP_SIZE / 16>()) {
+        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 16>());
+    } else if (warp_converged<WARP_SIZE / 32>()) {
+        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 32>());
+    } else {
+        return detail::divergent_load(src);
+    }
+}
+
+// Store data to *dest cooperatively with the widest converged tile.
+template <typename T>
+__device__ void store(const T& data, T* dest) {
+    if (warp_converged<WARP_SIZE>()) {
+        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE>());
+    } else if (warp_converged<WARP_SIZE / 2>()) {
+        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 2>());
+    } else if (warp_converged<WARP_SIZE / 4>()) {
+        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 4>());
+    } else if (warp_converged<WARP_SIZE / 8>()) {
+        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 8>());
+    } else if (warp_converged<WARP_SIZE / 16>()) {
+        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 16>());
+    } else if (warp_converged<WARP_SIZE / 32>()) {
+        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 32>());
+    } else {
+        detail::divergent_store(data, dest);
+    }
+}
+
+}  // namespace trove

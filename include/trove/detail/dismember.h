@@ -1,0 +1,56 @@
+// Splitting a struct into machine words and joining it back.
+//
+// Same rule as trove/detail/dismember.h:37-66: the word type U is the widest
+// of 16, 8 or 4 bytes that divides sizeof(T) (else 1 byte), and a T is
+// m = sizeof(T) / sizeof(U) words. On CDNA a 16-byte word is one
+// global_load_dwordx4 per lane; the transposition moves it as 4 dwords.
+// lyse / fuse (dismember.h:117-129) copy through memcpy, which the compiler
+// lowers to register renaming.
+#pragma once
+#include <trove/array.h>
+#include <trove/utility.h>
+#include <cstdint>
+#include <type_traits>
+
+namespace trove {
+namespace detail {
+
+struct alignas(8) word2 {
+    uint32_t x, y;
+};
+struct alignas(16) word4 {
+    uint32_t x, y, z, w;
+};
+
+template <typename T>
+struct dismember_type {
+    typedef typename std::conditional<
+        sizeof(T) % 16 == 0, word4,
+        typename std::conditional<sizeof(T) % 8 == 0, word2,
+                                  typename std::conditional<sizeof(T) % 4 == 0, uint32_t, char>::type>::type>::type
+        type;
+};
+
+template <typename T, typename U>
+struct aliased_size {
+    static_assert(sizeof(T) % sizeof(U) == 0, "struct size must be a multiple of the word size");
+    static const int value = sizeof(T) / sizeof(U);
+};
+
+template <typename U, typename T>
+__host__ __device__ array<U, aliased_size<T, U>::value> lyse(const T& in) {
+    array<U, aliased_size<T, U>::value> out;
+    static_assert(sizeof(out) == sizeof(T), "word array must alias the struct");
+    __builtin_memcpy(&out, &in, sizeof(T));
+    return out;
+}
+
+template <typename T, typename U>
+__host__ __device__ T fuse(const array<U, aliased_size<T, U>::value>& in) {
+    T out;
+    __builtin_memcpy(&out, &in, sizeof(T));
+    return out;
+}
+
+}  // namespace detail
+}  // namespace trove

@@ -1,0 +1,48 @@
+// coalesced_ptr<T>: a pointer whose element reads and writes are performed
+// cooperatively by the converged lanes around the caller.
+//
+// Public interface of trove/ptr.h:34-90. Reading a coalesced_ref calls
+// trove::load, assigning a T calls trove::store. Assigning one coalesced_ref
+// to another (dst[i] = src[j]) loads through trove::load and stores through
+// trove::store; the reference's version of that operator omits the tile
+// argument of its dispatch calls and cannot be instantiated (ptr.h:47-50).
+#pragma once
+#include <trove/aos.h>
+
+namespace trove {
+namespace detail {
+
+template <typename T>
+struct coalesced_ref {
+    T* m_ptr;
+    __device__ explicit coalesced_ref(T* ptr) : m_ptr(ptr) {}
+
+    __device__ operator T() { return trove::load(m_ptr); }
+
+    __device__ coalesced_ref& operator=(const T& data) {
+        trove::store(data, m_ptr);
+        return *this;
+    }
+
+    __device__ coalesced_ref& operator=(const coalesced_ref& other) {
+        const T data = trove::load(other.m_ptr);
+        trove::store(data, m_ptr);
+        return *this;
+    }
+};
+
+}  // namespace detail
+
+template <typename T>
+struct coalesced_ptr {
+    T* m_ptr;
+    __device__ coalesced_ptr(T* ptr) : m_ptr(ptr) {}
+    __device__ trove::detail::coalesced_ref<T> operator*() { return trove::detail::coalesced_ref<T>(m_ptr); }
+    template <typename I>
+    __device__ trove::detail::coalesced_ref<T> operator[](const I& idx) {
+        return trove::detail::coalesced_ref<T>(m_ptr + idx);
+    }
+    __device__ operator T*() { return m_ptr; }
+};
+
+}  // namespace trove

@@ -1,0 +1,206 @@
+"""Host-side interface to libtrove_amd, the MI355X AoS load/store library.
+
+The device-side drop-in for Trove is the header library under include/trove/;
+this module drives the C ABI of include/trove_amd.h (the launches the
+reference's tests/benchmark.cu and tests/block.cu perform) through ctypes.
+Buffers may be torch tensors on the GPU or raw device addresses (int). When
+no stream is given, work is enqueued on torch's current stream, so torch
+events and synchronisation see it.
+
+The library is required: if libtrove_amd.so has not been built, importing
+works but every call raises TroveUnavailable. There is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libtrove_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "trove_amd.h")
+
+OK = 0
+ERR_INVALID = -1
+ERR_SIZE = -2
+ERR_ALIGN = -3
+ERR_VARIANT = -4
+ERR_HIP = -5
+ERR_NOMEM = -6
+ERR_BUDGET = -7
+ERR_FULL = -8
+ERR_NOTFOUND = -9
+ERR_AGAIN = -10
+ERR_RANGE = -11
+
+VARIANT_AUTO = 0
+VARIANT_SHFL = 1
+VARIANT_DIRECT = 2
+VARIANT_TILE = 3
+
+VARIANTS = {"auto": VARIANT_AUTO, "shfl": VARIANT_SHFL, "direct": VARIANT_DIRECT, "tile": VARIANT_TILE}
+
+# Struct sizes compiled into the library (bytes).
+STRUCT_SIZES = tuple(range(4, 65, 4)) + (96, 128, 160, 192, 224, 256, 2, 6)
+
+
+class TroveUnavailable(RuntimeError):
+    """libtrove_amd.so is missing or cannot be loaded."""
+
+
+class TroveError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        self.code = code
+        msg = strerror(code) if _lib is not None else str(code)
+        if code == ERR_HIP and _lib is not None:
+            msg += f" (hipError_t {_lib.trove_last_hip_error()})"
+        super().__init__(f"{where}: {msg} [{code}]")
+
+
+_lib = None
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_u64 = ctypes.c_uint64
+
+_SIGNATURES = {
+    "trove_strerror": ([_int], ctypes.c_char_p),
+    "trove_last_hip_error": ([], _int),
+    "trove_version": ([], _int),
+    "trove_struct_size_supported": ([_int], _int),
+    "trove_struct_word_bytes": ([_int], _int),
+    "trove_copy": ([_vp, _vp, _i64, _int, _int, _int, _vp], _int),
+    "trove_gather": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
+    "trove_scatter": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
+    "trove_store_counting": ([_vp, _i64, _int, _int, _vp], _int),
+    "trove_load_sum": ([_vp, _vp, _i64, _int, _int, _vp], _int),
+    "trove_block_copy": ([_vp, _vp, _i64, _int, _int, _u64, _vp], _int),
+    "trove_block_write": ([_vp, _i64, _int, _vp], _int),
+    "trove_memcpy_ceiling": ([_vp, _vp, _i64, _vp], _int),
+    "trove_gen_words": ([_vp, _i64, _u64], _int),
+    "trove_gen_permutation": ([_vp, _i64, _u64], _int),
+    "trove_fill_words_device": ([_vp, _i64, _u64, _vp], _int),
+}
+
+
+def lib():
+    """The loaded library (ctypes.CDLL); raises TroveUnavailable if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TroveUnavailable(
+                f"{LIB_PATH} is not built; run `python -m trove_amd.build` (or __graft_entry__.build())")
+        try:
+            L = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise TroveUnavailable(f"cannot load {LIB_PATH}: {e}") from e
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def declared_functions(header: str = HEADER_PATH) -> list[str]:
+    """Names of the functions include/trove_amd.h declares."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(trove_[a-z0-9_]+)\s*\(", text)))
+
+
+def strerror(code: int) -> str:
+    return lib().trove_strerror(code).decode()
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != OK:
+        raise TroveError(rc, where)
+
+
+def _addr(x) -> int:
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    raise TypeError(f"expected a tensor or a device address, got {type(x).__name__}")
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        import torch
+
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _variant(v) -> int:
+    return VARIANTS[v] if isinstance(v, str) else int(v)
+
+
+def copy(src, dst, n: int, struct_bytes: int, variant="shfl", tile: int = 64, stream=None) -> None:
+    """dst[i] = src[i] for n structs (load/store_warp_contiguous<tile> or naive)."""
+    _check(lib().trove_copy(_addr(src), _addr(dst), n, struct_bytes, _variant(variant), tile, _stream(stream)),
+           "trove_copy")
+
+
+def gather(src, idx, dst, n: int, struct_bytes: int, variant="tile", stream=None) -> None:
+    """dst[i] = src[idx[i]] for i < n."""
+    _check(lib().trove_gather(_addr(src), _addr(idx), _addr(dst), n, struct_bytes, _variant(variant),
+                              _stream(stream)), "trove_gather")
+
+
+def scatter(src, idx, dst, n: int, struct_bytes: int, variant="tile", stream=None) -> None:
+    """dst[idx[i]] = src[i] for i < n (idx distinct)."""
+    _check(lib().trove_scatter(_addr(src), _addr(idx), _addr(dst), n, struct_bytes, _variant(variant),
+                               _stream(stream)), "trove_scatter")
+
+
+def store_counting(dst, n: int, struct_bytes: int, variant="shfl", stream=None) -> None:
+    _check(lib().trove_store_counting(_addr(dst), n, struct_bytes, _variant(variant), _stream(stream)),
+           "trove_store_counting")
+
+
+def load_sum(src, out, n: int, struct_bytes: int, variant="shfl", stream=None) -> None:
+    _check(lib().trove_load_sum(_addr(src), _addr(out), n, struct_bytes, _variant(variant), _stream(stream)),
+           "trove_load_sum")
+
+
+def block_copy(src, dst, n_elems: int, s: int, divergent: bool = False, seed: int = 0, stream=None) -> None:
+    _check(lib().trove_block_copy(_addr(src), _addr(dst), n_elems, s, int(bool(divergent)), seed,
+                                  _stream(stream)), "trove_block_copy")
+
+
+def block_write(dst, n_arrays: int, s: int, stream=None) -> None:
+    _check(lib().trove_block_write(_addr(dst), n_arrays, s, _stream(stream)), "trove_block_write")
+
+
+def memcpy_ceiling(src, dst, nbytes: int, stream=None) -> None:
+    _check(lib().trove_memcpy_ceiling(_addr(src), _addr(dst), nbytes, _stream(stream)), "trove_memcpy_ceiling")
+
+
+def fill_words_device(out, n: int, seed: int, stream=None) -> None:
+    _check(lib().trove_fill_words_device(_addr(out), n, seed, _stream(stream)), "trove_fill_words_device")
+
+
+def gen_words(n: int, seed: int):
+    """Host array of n uint32 equal to what fill_words_device writes."""
+    import numpy as np
+
+    out = np.empty(n, dtype=np.uint32)
+    _check(lib().trove_gen_words(out.ctypes.data, n, seed), "trove_gen_words")
+    return out
+
+
+def gen_permutation(n: int, seed: int):
+    """Host int32 array: a uniform random permutation of [0, n) (Fisher-Yates)."""
+    import numpy as np
+
+    out = np.empty(n, dtype=np.int32)
+    _check(lib().trove_gen_permutation(out.ctypes.data, n, seed), "trove_gen_permutation")
+    return out

@@ -1,0 +1,221 @@
+// extern "C" entry points of libtrove_amd: argument validation, dispatch by
+// struct size to the kernel translation units, error mapping.
+#include "common.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <random>
+
+namespace trove_amd {
+
+int block_copy_dispatch(const uint32_t* x, uint32_t* r, int64_t e, int s, int divergent, uint64_t seed,
+                        hipStream_t st);
+int block_write_dispatch(uint32_t* r, int64_t l, int s, hipStream_t st);
+int memcpy_ceiling_launch(const void* src, void* dst, int64_t bytes, hipStream_t s);
+int fill_words_launch(uint32_t* out, int64_t n, uint64_t seed, hipStream_t s);
+uint32_t host_word_hash(uint64_t seed, uint64_t i);
+
+static thread_local int g_last_hip_error = 0;
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return TROVE_OK;
+    g_last_hip_error = (int)e;
+    return TROVE_ERR_HIP;
+}
+
+int launch_status() { return hip_status(hipGetLastError()); }
+
+int device_cu_count() {
+    static thread_local int cached_dev = -1;
+    static thread_local int cached_cus = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (dev != cached_dev) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached_dev = dev;
+        cached_cus = cus;
+    }
+    return cached_cus;
+}
+
+// Blocks for `waves` wavefront work units: one wavefront per unit, capped at
+// TROVE_MAX_BLOCKS_PER_CU (default 16) resident-sized blocks per compute
+// unit, beyond which kernels stride over the remaining work.
+int64_t grid_for(int64_t waves) {
+    static const int64_t per_cu = [] {
+        const char* e = std::getenv("TROVE_MAX_BLOCKS_PER_CU");
+        const long v = e ? std::atol(e) : 0;
+        return (int64_t)(v > 0 ? v : 16);
+    }();
+    int64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t cap = per_cu * device_cu_count();
+    if (blocks > cap) blocks = cap;
+    return blocks < 1 ? 1 : blocks;
+}
+
+typedef int (*copy_fn)(const void*, void*, int64_t, int, int, hipStream_t);
+typedef int (*index_fn)(const void*, const int32_t*, void*, int64_t, int, hipStream_t);
+typedef int (*counting_fn)(void*, int64_t, int, hipStream_t);
+typedef int (*loadsum_fn)(const void*, uint32_t*, int64_t, int, hipStream_t);
+
+struct size_entry {
+    int bytes;
+    copy_fn copy;
+    index_fn gather;
+    index_fn scatter;
+    counting_fn counting;
+    loadsum_fn loadsum;
+};
+
+static const size_entry* find_size(int bytes) {
+    static const size_entry table[] = {
+#define TROVE_ENTRY(B) {B, &copy_impl<B>, &gather_impl<B>, &scatter_impl<B>, &counting_impl<B>, &loadsum_impl<B>},
+        TROVE_FOR_EACH_SIZE(TROVE_ENTRY)
+#undef TROVE_ENTRY
+    };
+    for (const size_entry& e : table)
+        if (e.bytes == bytes) return &e;
+    return nullptr;
+}
+
+static bool valid_variant(int v) { return v >= TROVE_VARIANT_AUTO && v <= TROVE_VARIANT_TILE; }
+
+}  // namespace trove_amd
+
+using namespace trove_amd;
+
+extern "C" {
+
+const char* trove_strerror(int code) {
+    switch (code) {
+        case TROVE_OK: return "success";
+        case TROVE_ERR_INVALID: return "invalid argument";
+        case TROVE_ERR_SIZE: return "struct size not supported";
+        case TROVE_ERR_ALIGN: return "pointer not aligned to the struct's word size";
+        case TROVE_ERR_VARIANT: return "unknown variant or tile width";
+        case TROVE_ERR_HIP: return "HIP runtime error";
+        case TROVE_ERR_NOMEM: return "out of memory";
+        case TROVE_ERR_BUDGET: return "request exceeds the memory budget";
+        case TROVE_ERR_FULL: return "no free request slot";
+        case TROVE_ERR_NOTFOUND: return "unknown handle or request";
+        case TROVE_ERR_AGAIN: return "not finished";
+        case TROVE_ERR_RANGE: return "index out of range";
+        default: return "unknown error";
+    }
+}
+
+int trove_last_hip_error(void) { return g_last_hip_error; }
+
+int trove_version(void) { return 10000; }
+
+int trove_struct_size_supported(int struct_bytes) { return find_size(struct_bytes) != nullptr; }
+
+int trove_struct_word_bytes(int struct_bytes) { return struct_bytes > 0 ? word_bytes_of(struct_bytes) : 0; }
+
+int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int variant, int tile, void* stream) {
+    const size_entry* e = find_size(struct_bytes);
+    if (!e) return TROVE_ERR_SIZE;
+    if (n < 0) return TROVE_ERR_INVALID;
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE) return TROVE_ERR_VARIANT;
+    if (tile != 0 && (tile < 2 || tile > 64 || (tile & (tile - 1)) != 0)) return TROVE_ERR_VARIANT;
+    if (n == 0) return TROVE_OK;
+    if (!src || !dst) return TROVE_ERR_INVALID;
+    const int w = word_bytes_of(struct_bytes);
+    if (!aligned(src, w) || !aligned(dst, w)) return TROVE_ERR_ALIGN;
+    return e->copy(src, dst, n, variant, tile, static_cast<hipStream_t>(stream));
+}
+
+static int index_common(const size_entry* e, const void* src, const int32_t* idx, void* dst, int64_t n,
+                        int struct_bytes, int variant) {
+    if (!e) return TROVE_ERR_SIZE;
+    if (n < 0) return TROVE_ERR_INVALID;
+    if (!valid_variant(variant)) return TROVE_ERR_VARIANT;
+    if (n == 0) return TROVE_OK + 1;  // nothing to launch
+    if (!src || !dst || !idx) return TROVE_ERR_INVALID;
+    const int w = word_bytes_of(struct_bytes);
+    if (!aligned(src, w) || !aligned(dst, w) || !aligned(idx, 4)) return TROVE_ERR_ALIGN;
+    return TROVE_OK;
+}
+
+int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
+                 void* stream) {
+    const size_entry* e = find_size(struct_bytes);
+    const int rc = index_common(e, src, idx, dst, n, struct_bytes, variant);
+    if (rc != TROVE_OK) return rc > 0 ? TROVE_OK : rc;
+    return e->gather(src, idx, dst, n, variant, static_cast<hipStream_t>(stream));
+}
+
+int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
+                  void* stream) {
+    const size_entry* e = find_size(struct_bytes);
+    const int rc = index_common(e, src, idx, dst, n, struct_bytes, variant);
+    if (rc != TROVE_OK) return rc > 0 ? TROVE_OK : rc;
+    return e->scatter(src, idx, dst, n, variant, static_cast<hipStream_t>(stream));
+}
+
+int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, void* stream) {
+    const size_entry* e = find_size(struct_bytes);
+    if (!e || struct_bytes % 4 != 0) return TROVE_ERR_SIZE;
+    if (n < 0) return TROVE_ERR_INVALID;
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE) return TROVE_ERR_VARIANT;
+    if (n == 0) return TROVE_OK;
+    if (!dst) return TROVE_ERR_INVALID;
+    if (!aligned(dst, word_bytes_of(struct_bytes))) return TROVE_ERR_ALIGN;
+    return e->counting(dst, n, variant, static_cast<hipStream_t>(stream));
+}
+
+int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, int variant, void* stream) {
+    const size_entry* e = find_size(struct_bytes);
+    if (!e || struct_bytes % 4 != 0) return TROVE_ERR_SIZE;
+    if (n < 0) return TROVE_ERR_INVALID;
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE) return TROVE_ERR_VARIANT;
+    if (n == 0) return TROVE_OK;
+    if (!src || !out) return TROVE_ERR_INVALID;
+    if (!aligned(src, word_bytes_of(struct_bytes)) || !aligned(out, 4)) return TROVE_ERR_ALIGN;
+    return e->loadsum(src, out, n, variant, static_cast<hipStream_t>(stream));
+}
+
+int trove_block_copy(const void* src, void* dst, int64_t n_elems, int s, int divergent, uint64_t seed,
+                     void* stream) {
+    if (s < 1 || s > 16 || n_elems < 0) return TROVE_ERR_INVALID;
+    if (n_elems == 0) return TROVE_OK;
+    if (!src || !dst) return TROVE_ERR_INVALID;
+    const int w = word_bytes_of(4 * s);
+    if (!aligned(src, w) || !aligned(dst, w)) return TROVE_ERR_ALIGN;
+    return block_copy_dispatch(static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), n_elems, s,
+                               divergent, seed, static_cast<hipStream_t>(stream));
+}
+
+int trove_block_write(void* dst, int64_t n_arrays, int s, void* stream) {
+    if (s < 1 || s > 16 || n_arrays < 0) return TROVE_ERR_INVALID;
+    if (n_arrays == 0) return TROVE_OK;
+    if (!dst) return TROVE_ERR_INVALID;
+    if (!aligned(dst, word_bytes_of(4 * s))) return TROVE_ERR_ALIGN;
+    return block_write_dispatch(static_cast<uint32_t*>(dst), n_arrays, s, static_cast<hipStream_t>(stream));
+}
+
+int trove_memcpy_ceiling(const void* src, void* dst, int64_t bytes, void* stream) {
+    if (bytes < 0 || bytes % 16 != 0) return TROVE_ERR_INVALID;
+    if (bytes == 0) return TROVE_OK;
+    if (!src || !dst) return TROVE_ERR_INVALID;
+    if (!aligned(src, 16) || !aligned(dst, 16)) return TROVE_ERR_ALIGN;
+    return memcpy_ceiling_launch(src, dst, bytes, static_cast<hipStream_t>(stream));
+}
+
+int trove_gen_words(uint32_t* out, int64_t n, uint64_t seed) {
+    if (n < 0 || (n > 0 && !out)) return TROVE_ERR_INVALID;
+    for (int64_t i = 0; i < n; ++i) out[i] = host_word_hash(seed, (uint64_t)i);
+    return TROVE_OK;
+}
+
+int trove_fill_words_device(uint32_t* out, int64_t n, uint64_t seed, void* stream) {
+    if (n < 0) return TROVE_ERR_INVALID;
+    if (n == 0) return TROVE_OK;
+    if (!out) return TROVE_ERR_INVALID;
+    if (!aligned(out, 4)) return TROVE_ERR_ALIGN;
+    return fill_words_launch(out, n, seed, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// Internal helpers shared by the kernel translation units of libtrove_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <trove/aos.h>
+#include <trove/block.h>
+#include <trove/ptr.h>
+#include <trove_amd.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace trove_amd {
+
+// Work-group size of every kernel: four wavefronts.
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+
+// The struct type moved for a given byte size: k 32-bit words when the size
+// is a multiple of 4 (the reference benchmark's trove::array<int, k>),
+// otherwise raw bytes.
+template <int B>
+using blob_t = typename std::conditional<B % 4 == 0, trove::array<uint32_t, (B % 4 == 0 ? B / 4 : 1)>,
+                                         trove::array<uint8_t, B> >::type;
+
+// Struct sizes compiled into the C ABI: the 4..64-byte sweep, larger sizes
+// that reach the composite schedule (96, 160, 192, 224 B: 6, 10, 12, 14
+// 16-byte words) and two sizes that are not a multiple of 4. They are split
+// into shards so that the kernel translation units can be compiled in
+// parallel: a unit built with -DTROVE_SHARD=k instantiates shard k only
+// (without it, all shards).
+#define TROVE_SIZES_SHARD0(X) X(4) X(8) X(12) X(16) X(20) X(224)
+#define TROVE_SIZES_SHARD1(X) X(24) X(28) X(32) X(36) X(256)
+#define TROVE_SIZES_SHARD2(X) X(40) X(44) X(48) X(2) X(192)
+#define TROVE_SIZES_SHARD3(X) X(52) X(56) X(6) X(160)
+#define TROVE_SIZES_SHARD4(X) X(60) X(64) X(128)
+#define TROVE_SIZES_SHARD5(X) X(96)
+#define TROVE_SIZE_SHARDS 6
+#define TROVE_FOR_EACH_SIZE(X)                                                                                 \
+    TROVE_SIZES_SHARD0(X) TROVE_SIZES_SHARD1(X) TROVE_SIZES_SHARD2(X) TROVE_SIZES_SHARD3(X) TROVE_SIZES_SHARD4(X) \
+        TROVE_SIZES_SHARD5(X)
+
+#if !defined(TROVE_SHARD)
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_FOR_EACH_SIZE(X)
+#elif TROVE_SHARD == 0
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD0(X)
+#elif TROVE_SHARD == 1
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD1(X)
+#elif TROVE_SHARD == 2
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD2(X)
+#elif TROVE_SHARD == 3
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD3(X)
+#elif TROVE_SHARD == 4
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD4(X)
+#elif TROVE_SHARD == 5
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD5(X)
+#endif
+
+// Global wavefront index and count (scalar: the same on every lane).
+__device__ __forceinline__ int64_t wave_index() {
+    return (int64_t)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+__device__ __forceinline__ int64_t wave_count() { return (int64_t)gridDim.x * kWavesPerBlock; }
+
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Host side: launch bookkeeping and error reporting.
+int hip_status(hipError_t e);         // maps to TROVE_OK / TROVE_ERR_HIP and records e
+int launch_status();                  // checks hipGetLastError after a launch
+int device_cu_count();                // compute units of the current device
+int64_t grid_for(int64_t work_items); // blocks for `work_items` wavefront work units, capped
+
+inline bool aligned(const void* p, int bytes) { return ((uintptr_t)p % (uintptr_t)bytes) == 0; }
+
+template <int B>
+constexpr int word_bytes() {
+    return B % 16 == 0 ? 16 : B % 8 == 0 ? 8 : B % 4 == 0 ? 4 : 1;
+}
+
+inline int word_bytes_of(int b) { return b % 16 == 0 ? 16 : b % 8 == 0 ? 8 : b % 4 == 0 ? 4 : 1; }
+
+// Per-size entry points, defined in the kernel translation units.
+template <int B>
+int copy_impl(const void* src, void* dst, int64_t n, int variant, int tile, hipStream_t s);
+template <int B>
+int gather_impl(const void* src, const int32_t* idx, void* dst, int64_t n, int variant, hipStream_t s);
+template <int B>
+int scatter_impl(const void* src, const int32_t* idx, void* dst, int64_t n, int variant, hipStream_t s);
+template <int B>
+int counting_impl(void* dst, int64_t n, int variant, hipStream_t s);
+template <int B>
+int loadsum_impl(const void* src, uint32_t* out, int64_t n, int variant, hipStream_t s);
+
+}  // namespace trove_amd
