@@ -109,6 +109,80 @@ int trove_block_write(void* dst, int64_t n_arrays, int s, void* stream);
  * the practical bandwidth ceiling the AoS paths are compared against. */
 int trove_memcpy_ceiling(const void* src, void* dst, int64_t bytes, void* stream);
 
+/* ---- Batched gather: many independent jobs in one persistent launch ----
+ * No reference counterpart (the reference launches one kernel per
+ * benchmark case, tests/benchmark.cu:143-145); each job is the gather of
+ * trove_gather. */
+typedef struct trove_job {
+    const void* src;      /* device: source structs */
+    const int32_t* idx;   /* device: n indices into src */
+    void* dst;            /* device: n output structs */
+    int64_t n;            /* structs to gather */
+    int32_t struct_bytes; /* 4..64, a multiple of 4 */
+    int32_t variant;      /* TROVE_VARIANT_TILE / SHFL / DIRECT (AUTO = TILE) */
+    int32_t chunk_groups; /* 64-struct groups per work item (0 = 16) */
+    int32_t reserved;     /* must be 0 */
+} trove_job;
+
+typedef struct trove_batch trove_batch; /* opaque plan */
+
+#define TROVE_BATCH_PERSISTENT 0 /* one launch walking every job's work items */
+#define TROVE_BATCH_GRAPH 1      /* that launch captured once as a hipGraph, then replayed */
+#define TROVE_BATCH_LOOPED 2     /* one trove_gather launch per job (the comparison) */
+
+/* Validates the jobs and uploads the job table (allocates device memory). */
+int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out);
+/* Enqueues the whole batch on `stream` in the given mode. */
+int trove_batch_run(trove_batch* b, int mode, void* stream);
+/* Work items and algorithmic bytes ((2 * struct_bytes + 4) * n summed). */
+int trove_batch_stats(const trove_batch* b, int64_t* work_items, int64_t* algorithmic_bytes);
+int trove_batch_destroy(trove_batch* b);
+
+/* ---- Serving: continuously batched gathers against a device table ----
+ * No reference counterpart. Requests "gather rows idx[0..n) of the table"
+ * are admitted in FIFO order while a request slot and `n` rows of the
+ * output budget are free; every step gathers the next slice (at most
+ * slice_indices rows) of every admitted request in one launch, so requests
+ * that arrive while others run join the next step. A finished request's
+ * rows stay in the device output arena until trove_serve_release. One host
+ * thread drives a server; calls on one server must not overlap. */
+typedef struct trove_serve trove_serve; /* opaque server */
+
+/* table: device rows of row_bytes bytes (4..64, multiple of 4), n_rows of
+ * them. budget_bytes bounds the output rows of admitted requests (the index
+ * arena is budget_bytes / row_bytes * 4 bytes more). Work runs on `stream`;
+ * host index uploads run on a private stream. */
+int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int64_t budget_bytes,
+                       int32_t max_slots, int64_t slice_indices, int32_t variant, void* stream,
+                       trove_serve** out);
+/* Queues a request. Host indices are range-checked (TROVE_ERR_RANGE) and
+ * must stay valid until the request is done; device indices are trusted.
+ * TROVE_ERR_BUDGET when n rows exceed the whole budget. */
+int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on_device, uint64_t* request_id);
+/* Retires finished steps, admits queued requests and launches one step.
+ * *in_flight = requests queued or running plus steps outstanding. */
+int trove_serve_step(trove_serve* s, int32_t* in_flight);
+/* *done = 1 once the request's rows are written; latency from submission. */
+int trove_serve_poll(trove_serve* s, uint64_t request_id, int32_t* done, double* latency_s);
+/* Device address of a finished request's n output rows (until released). */
+int trove_serve_output(trove_serve* s, uint64_t request_id, void** dev_out);
+/* Copies a finished request's n rows to `out` (host or device memory),
+ * synchronously. */
+int trove_serve_fetch(trove_serve* s, uint64_t request_id, void* out);
+/* Frees a finished request's rows (the caller's reads must be complete). */
+int trove_serve_release(trove_serve* s, uint64_t request_id);
+/* Steps until every submitted request is done (blocking). */
+int trove_serve_drain(trove_serve* s);
+int trove_serve_stats(const trove_serve* s, int64_t* completed, int64_t* indices_done, int64_t* steps,
+                      int32_t* queued, int32_t* active);
+/* Replays an arrival trace in real time: request i (idx[i], n[i] rows) is
+ * submitted at arrival_s[i] seconds after the start (ascending); returns
+ * when all are done, with latency_s[i] = completion - scheduled arrival.
+ * release_on_done frees each request's rows as soon as it completes. */
+int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
+                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s);
+int trove_serve_destroy(trove_serve* s);
+
 /* Deterministic inputs (host memory; no GPU needed).
  * trove_gen_words: out[i] = 32-bit hash of (seed, i).
  * trove_gen_permutation: uniform random permutation of [0, n) (Fisher-Yates,
@@ -117,6 +191,17 @@ int trove_gen_words(uint32_t* out, int64_t n, uint64_t seed);
 int trove_gen_permutation(int32_t* out, int64_t n, uint64_t seed);
 /* Device fill with the same values as trove_gen_words (stream-ordered). */
 int trove_fill_words_device(uint32_t* out, int64_t n, uint64_t seed, void* stream);
+
+#define TROVE_PATTERN_IDENTITY 1 /* out[i] = i % n_src */
+#define TROVE_PATTERN_RANDOM 2   /* a permutation of [0, n) when n == n_src, else uniform draws */
+#define TROVE_PATTERN_RUNS 3     /* ascending runs of geometric length (mean 32) at uniform starts */
+/* n indices into [0, n_src) following `pattern`. */
+int trove_gen_indices(int32_t* out, int64_t n, int64_t n_src, int pattern, uint64_t seed);
+/* n draws from Zipf(s) over ranks 1..n_items, returned as rows rank - 1
+ * (rejection-inversion, Hoermann & Derflinger 1996). */
+int trove_gen_zipf(int32_t* out, int64_t n, int64_t n_items, double s, uint64_t seed);
+/* n integers log-uniform on [lo, hi]: floor(exp(U(log lo, log(hi + 1)))). */
+int trove_gen_loguniform(int64_t* out, int64_t n, int64_t lo, int64_t hi, uint64_t seed);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,25 @@ _SIGNATURES = {
     "trove_gen_words": ([_vp, _i64, _u64], _int),
     "trove_gen_permutation": ([_vp, _i64, _u64], _int),
     "trove_fill_words_device": ([_vp, _i64, _u64, _vp], _int),
+    "trove_gen_indices": ([_vp, _i64, _i64, _int, _u64], _int),
+    "trove_gen_zipf": ([_vp, _i64, _i64, ctypes.c_double, _u64], _int),
+    "trove_gen_loguniform": ([_vp, _i64, _i64, _i64, _u64], _int),
+    "trove_batch_create": ([_vp, ctypes.c_int32, _vp], _int),
+    "trove_batch_run": ([_vp, _int, _vp], _int),
+    "trove_batch_stats": ([_vp, _vp, _vp], _int),
+    "trove_batch_destroy": ([_vp], _int),
+    "trove_serve_create": ([_vp, _i64, ctypes.c_int32, _i64, ctypes.c_int32, _i64, ctypes.c_int32, _vp, _vp],
+                           _int),
+    "trove_serve_submit": ([_vp, _vp, _i64, _int, _vp], _int),
+    "trove_serve_step": ([_vp, _vp], _int),
+    "trove_serve_poll": ([_vp, _u64, _vp, _vp], _int),
+    "trove_serve_output": ([_vp, _u64, _vp], _int),
+    "trove_serve_release": ([_vp, _u64], _int),
+    "trove_serve_fetch": ([_vp, _u64, _vp], _int),
+    "trove_serve_drain": ([_vp], _int),
+    "trove_serve_stats": ([_vp, _vp, _vp, _vp, _vp, _vp], _int),
+    "trove_serve_replay": ([_vp, ctypes.c_int32, _vp, _vp, _vp, _int, _int, _vp], _int),
+    "trove_serve_destroy": ([_vp], _int),
 }
 
 
@@ -204,3 +223,162 @@ def gen_permutation(n: int, seed: int):
     out = np.empty(n, dtype=np.int32)
     _check(lib().trove_gen_permutation(out.ctypes.data, n, seed), "trove_gen_permutation")
     return out
+
+
+PATTERN_IDENTITY = 1
+PATTERN_RANDOM = 2
+PATTERN_RUNS = 3
+PATTERNS = {"identity": PATTERN_IDENTITY, "random": PATTERN_RANDOM, "runs": PATTERN_RUNS}
+
+
+def gen_indices(n: int, n_src: int, pattern, seed: int):
+    """Host int32 indices into [0, n_src): identity, random (a permutation when
+    n == n_src) or ascending runs of geometric length (mean 32)."""
+    import numpy as np
+
+    pat = PATTERNS[pattern] if isinstance(pattern, str) else int(pattern)
+    out = np.empty(n, dtype=np.int32)
+    _check(lib().trove_gen_indices(out.ctypes.data, n, n_src, pat, seed), "trove_gen_indices")
+    return out
+
+
+def gen_zipf(n: int, n_items: int, s: float, seed: int):
+    """Host int32 rows drawn Zipf(s) over ranks 1..n_items (row = rank - 1)."""
+    import numpy as np
+
+    out = np.empty(n, dtype=np.int32)
+    _check(lib().trove_gen_zipf(out.ctypes.data, n, n_items, float(s), seed), "trove_gen_zipf")
+    return out
+
+
+def gen_loguniform(n: int, lo: int, hi: int, seed: int):
+    import numpy as np
+
+    out = np.empty(n, dtype=np.int64)
+    _check(lib().trove_gen_loguniform(out.ctypes.data, n, lo, hi, seed), "trove_gen_loguniform")
+    return out
+
+
+class Job(ctypes.Structure):
+    """Mirror of struct trove_job."""
+    _fields_ = [("src", ctypes.c_void_p), ("idx", ctypes.c_void_p), ("dst", ctypes.c_void_p),
+                ("n", ctypes.c_int64), ("struct_bytes", ctypes.c_int32), ("variant", ctypes.c_int32),
+                ("chunk_groups", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+BATCH_PERSISTENT = 0
+BATCH_GRAPH = 1
+BATCH_LOOPED = 2
+BATCH_MODES = {"persistent": BATCH_PERSISTENT, "graph": BATCH_GRAPH, "looped": BATCH_LOOPED}
+
+
+class Batch:
+    """A validated set of gather jobs runnable as one persistent launch."""
+
+    def __init__(self, jobs):
+        """jobs: iterable of (src, idx, dst, n, struct_bytes, variant, chunk_groups)."""
+        arr = (Job * len(jobs))()
+        for j, (src, idx, dst, n, b, variant, chunk) in enumerate(jobs):
+            arr[j] = Job(_addr(src), _addr(idx), _addr(dst), n, b, _variant(variant), chunk, 0)
+        self._jobs = arr
+        h = ctypes.c_void_p()
+        _check(lib().trove_batch_create(ctypes.addressof(arr), len(jobs), ctypes.byref(h)), "trove_batch_create")
+        self._h = h
+
+    def run(self, mode="persistent", stream=None) -> None:
+        m = BATCH_MODES[mode] if isinstance(mode, str) else int(mode)
+        _check(lib().trove_batch_run(self._h, m, _stream(stream)), "trove_batch_run")
+
+    def stats(self):
+        items, nbytes = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().trove_batch_stats(self._h, ctypes.byref(items), ctypes.byref(nbytes)), "trove_batch_stats")
+        return items.value, nbytes.value
+
+    def close(self) -> None:
+        if self._h:
+            lib().trove_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuously batched gather server (trove_serve_*)."""
+
+    def __init__(self, table, n_rows: int, row_bytes: int, budget_bytes: int, max_slots: int = 1024,
+                 slice_indices: int = 1 << 16, variant="tile", stream=None):
+        h = ctypes.c_void_p()
+        self.row_bytes = row_bytes
+        _check(lib().trove_serve_create(_addr(table), n_rows, row_bytes, budget_bytes, max_slots, slice_indices,
+                                        _variant(variant), _stream(stream), ctypes.byref(h)), "trove_serve_create")
+        self._h = h
+
+    def submit(self, idx, n: int, on_device: bool) -> int:
+        rid = ctypes.c_uint64()
+        ptr = _addr(idx) if on_device else idx.ctypes.data
+        _check(lib().trove_serve_submit(self._h, ptr, n, int(on_device), ctypes.byref(rid)), "trove_serve_submit")
+        return rid.value
+
+    def step(self) -> int:
+        inflight = ctypes.c_int32()
+        _check(lib().trove_serve_step(self._h, ctypes.byref(inflight)), "trove_serve_step")
+        return inflight.value
+
+    def poll(self, rid: int):
+        done, lat = ctypes.c_int32(), ctypes.c_double()
+        _check(lib().trove_serve_poll(self._h, rid, ctypes.byref(done), ctypes.byref(lat)), "trove_serve_poll")
+        return bool(done.value), lat.value
+
+    def output(self, rid: int) -> int:
+        p = ctypes.c_void_p()
+        _check(lib().trove_serve_output(self._h, rid, ctypes.byref(p)), "trove_serve_output")
+        return p.value or 0
+
+    def fetch(self, rid: int, n: int):
+        """Host copy (uint8, n * row_bytes) of a finished request's rows."""
+        import numpy as np
+
+        out = np.empty(n * self.row_bytes, dtype=np.uint8)
+        _check(lib().trove_serve_fetch(self._h, rid, out.ctypes.data), "trove_serve_fetch")
+        return out
+
+    def release(self, rid: int) -> None:
+        _check(lib().trove_serve_release(self._h, rid), "trove_serve_release")
+
+    def drain(self) -> None:
+        _check(lib().trove_serve_drain(self._h), "trove_serve_drain")
+
+    def stats(self):
+        c, i, st = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        q, a = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().trove_serve_stats(self._h, *(ctypes.byref(x) for x in (c, i, st, q, a))), "trove_serve_stats")
+        return {"completed": c.value, "indices_done": i.value, "steps": st.value, "queued": q.value,
+                "active": a.value}
+
+    def replay(self, arrival_s, idx_ptrs, counts, on_device: bool, release_on_done: bool = True):
+        """Real-time replay of an arrival trace; returns per-request latency (s)."""
+        import numpy as np
+
+        n = len(counts)
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        ptrs = np.ascontiguousarray(idx_ptrs, dtype=np.uint64)
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        lat = np.zeros(n, dtype=np.float64)
+        _check(lib().trove_serve_replay(self._h, n, arr.ctypes.data, ptrs.ctypes.data, cnt.ctypes.data,
+                                        int(on_device), int(release_on_done), lat.ctypes.data), "trove_serve_replay")
+        return lat
+
+    def close(self) -> None:
+        if self._h:
+            lib().trove_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

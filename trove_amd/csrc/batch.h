@@ -1,0 +1,28 @@
+// Job table of the persistent batch-gather kernel (batch.hip), shared with
+// the serving engine, which launches one table per step.
+#pragma once
+#include "common.h"
+
+namespace trove_amd {
+
+// One gather job as the kernel sees it. Work item t of the launch belongs to
+// the job with first_item <= t < next job's first_item and covers
+// chunk_groups 64-struct groups of it.
+struct job_desc {
+    const void* src;
+    const int32_t* idx;
+    void* dst;
+    int64_t n;
+    int64_t first_item;
+    int32_t bytes;  // 4..64, a multiple of 4
+    int32_t variant;
+    int32_t chunk_groups;
+    int32_t pad;
+};
+static_assert(sizeof(job_desc) == 56, "job_desc layout");
+
+// Launches the persistent gather over a device job table (sorted by
+// first_item, total_items work items in all).
+int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, hipStream_t s);
+
+}  // namespace trove_amd

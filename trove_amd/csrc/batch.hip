@@ -1,0 +1,203 @@
+// Batched gather: many independent jobs dst_j[i] = src_j[idx_j[i]] of
+// different struct sizes and lengths in ONE persistent launch.
+//
+// Work is cut into items of `chunk_groups` 64-struct groups of one job; item
+// t belongs to the job j with first_item[j] <= t < first_item[j + 1]. Every
+// wavefront walks the item space with a grid stride, finds its job by a
+// binary search over the (wave-uniform, scalar-loaded) job table and runs the
+// job's variant for its struct size. The launch can be captured once into a
+// hipGraph and replayed. The looped mode launches the standalone gather
+// kernel once per job on the same stream, for comparison at equal output.
+#include "batch.h"
+#include "gather_body.h"
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+namespace trove_amd {
+
+#define TROVE_BATCH_SIZES(X) \
+    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40) X(44) X(48) X(52) X(56) X(60) X(64)
+
+template <int B>
+__device__ __forceinline__ void batch_item(const job_desc& J, int64_t local) {
+    typedef blob_t<B> T;
+    const T* src = static_cast<const T*>(J.src);
+    T* dst = static_cast<T*>(J.dst);
+    const int64_t groups = (J.n + 63) >> 6;
+    const int64_t g0 = local * J.chunk_groups;
+    const int64_t g1 = g0 + J.chunk_groups < groups ? g0 + J.chunk_groups : groups;
+    if constexpr (trove::detail::use_shfl<T, 64>::value) {
+        if (J.variant == TROVE_VARIANT_TILE) {
+            gather_range_tile<T, gather_groups_in_flight<T>()>(src, J.idx, dst, J.n, g0, g1);
+            return;
+        }
+        if (J.variant == TROVE_VARIANT_SHFL) {
+            gather_range_shfl<T>(src, J.idx, dst, J.n, g0, g1);
+            return;
+        }
+    }
+    gather_range_direct<T>(src, J.idx, dst, J.n, g0, g1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_gather(const job_desc* __restrict__ jobs, int n_jobs,
+                                                         int64_t total_items) {
+    for (int64_t it = wave_index(); it < total_items; it += wave_count()) {
+        int lo = 0, hi = n_jobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].first_item <= it) {
+                lo = mid;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        const job_desc J = jobs[lo];
+        const int64_t local = it - J.first_item;
+        switch (J.bytes) {
+#define TROVE_BATCH_CASE(B)    \
+    case B:                    \
+        batch_item<B>(J, local); \
+        break;
+            TROVE_BATCH_SIZES(TROVE_BATCH_CASE)
+#undef TROVE_BATCH_CASE
+            default:
+                break;
+        }
+    }
+}
+
+int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, hipStream_t s) {
+    const int64_t blocks = grid_for(total_items > 0 ? total_items : 1);
+    hipLaunchKernelGGL(k_batch_gather, dim3((unsigned)blocks), dim3(kBlock), 0, s, d_table, n_jobs, total_items);
+    return launch_status();
+}
+
+}  // namespace trove_amd
+
+using namespace trove_amd;
+
+struct trove_batch {
+    std::vector<trove_job> jobs;
+    std::vector<job_desc> table;
+    job_desc* d_table = nullptr;
+    int64_t total_items = 0;
+    int64_t algorithmic_bytes = 0;
+    int device = 0;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+static int batch_launch(trove_batch* b, hipStream_t s) {
+    return launch_job_table(b->d_table, (int)b->table.size(), b->total_items, s);
+}
+
+extern "C" {
+
+int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out) {
+    if (!out || n_jobs <= 0 || !jobs) return TROVE_ERR_INVALID;
+    *out = nullptr;
+    trove_batch* b = new (std::nothrow) trove_batch();
+    if (!b) return TROVE_ERR_NOMEM;
+    b->jobs.assign(jobs, jobs + n_jobs);
+    int64_t item = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        const trove_job& J = jobs[j];
+        const int bytes = J.struct_bytes;
+        if (bytes < 4 || bytes > 64 || bytes % 4 != 0) {
+            delete b;
+            return TROVE_ERR_SIZE;
+        }
+        if (J.n < 0 || (J.n > 0 && (!J.src || !J.idx || !J.dst))) {
+            delete b;
+            return TROVE_ERR_INVALID;
+        }
+        if (J.variant < TROVE_VARIANT_AUTO || J.variant > TROVE_VARIANT_TILE || J.chunk_groups < 0) {
+            delete b;
+            return TROVE_ERR_VARIANT;
+        }
+        const int w = word_bytes_of(bytes);
+        if (!aligned(J.src, w) || !aligned(J.dst, w) || !aligned(J.idx, 4)) {
+            delete b;
+            return TROVE_ERR_ALIGN;
+        }
+        job_desc d{};
+        d.src = J.src;
+        d.idx = J.idx;
+        d.dst = J.dst;
+        d.n = J.n;
+        d.bytes = bytes;
+        d.variant = J.variant == TROVE_VARIANT_AUTO ? TROVE_VARIANT_TILE : J.variant;
+        d.chunk_groups = J.chunk_groups > 0 ? J.chunk_groups : 16;
+        d.first_item = item;
+        const int64_t groups = (J.n + 63) / 64;
+        item += (groups + d.chunk_groups - 1) / d.chunk_groups;
+        b->algorithmic_bytes += (2 * (int64_t)bytes + 4) * J.n;
+        b->table.push_back(d);
+    }
+    b->total_items = item;
+    int rc = hip_status(hipGetDevice(&b->device));
+    if (rc == TROVE_OK) rc = hip_status(hipMalloc(&b->d_table, sizeof(job_desc) * b->table.size()));
+    if (rc == TROVE_OK)
+        rc = hip_status(
+            hipMemcpy(b->d_table, b->table.data(), sizeof(job_desc) * b->table.size(), hipMemcpyHostToDevice));
+    if (rc != TROVE_OK) {
+        if (b->d_table) (void)hipFree(b->d_table);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return TROVE_OK;
+}
+
+int trove_batch_run(trove_batch* b, int mode, void* stream) {
+    if (!b) return TROVE_ERR_INVALID;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (b->total_items == 0) return TROVE_OK;
+    if (mode == TROVE_BATCH_PERSISTENT) return batch_launch(b, s);
+    if (mode == TROVE_BATCH_GRAPH) {
+        if (!b->graph_exec) {
+            int rc = hip_status(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+            if (rc != TROVE_OK) return rc;
+            rc = hip_status(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+            if (rc != TROVE_OK) return rc;
+            const int lrc = batch_launch(b, b->capture_stream);
+            rc = hip_status(hipStreamEndCapture(b->capture_stream, &b->graph));
+            if (lrc != TROVE_OK) return lrc;
+            if (rc != TROVE_OK) return rc;
+            rc = hip_status(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+            if (rc != TROVE_OK) return rc;
+        }
+        return hip_status(hipGraphLaunch(b->graph_exec, s));
+    }
+    if (mode == TROVE_BATCH_LOOPED) {
+        for (const job_desc& d : b->table) {
+            if (d.n == 0) continue;
+            const int rc = trove_gather(d.src, d.idx, d.dst, d.n, d.bytes, d.variant, stream);
+            if (rc != TROVE_OK) return rc;
+        }
+        return TROVE_OK;
+    }
+    return TROVE_ERR_INVALID;
+}
+
+int trove_batch_stats(const trove_batch* b, int64_t* work_items, int64_t* algorithmic_bytes) {
+    if (!b) return TROVE_ERR_INVALID;
+    if (work_items) *work_items = b->total_items;
+    if (algorithmic_bytes) *algorithmic_bytes = b->algorithmic_bytes;
+    return TROVE_OK;
+}
+
+int trove_batch_destroy(trove_batch* b) {
+    if (!b) return TROVE_ERR_INVALID;
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    if (b->d_table) (void)hipFree(b->d_table);
+    delete b;
+    return TROVE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Per-wavefront gather bodies over a range of 64-struct groups, shared by the
+// standalone gather kernels and the persistent batch kernel.
+//
+// A wavefront calls one of these with the same [g0, g1) on every lane;
+// group g covers structs [64 g, 64 g + 64) of a job of n structs, and the last
+// group may be partial (n % 64 lanes).
+#pragma once
+#include "common.h"
+
+namespace trove_amd {
+
+template <typename T>
+constexpr int gather_groups_in_flight() {
+    return sizeof(T) <= 16 ? 4 : sizeof(T) <= 32 ? 2 : 1;
+}
+
+// Tile path: indexed word loads in column-side layout (each word's owner
+// index broadcast by ds_bpermute), then coalesced tile stores; no
+// transposition. UNROLL groups are in flight per iteration.
+template <typename T, int UNROLL>
+__device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                  T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1) {
+    typedef trove::detail::word_layout<T> L;
+    const int lane = trove::lane_id();
+    const int64_t full = n >> 6;
+    const int64_t full_end = g1 < full ? g1 : full;
+    int64_t g = g0;
+    for (; g + UNROLL <= full_end; g += UNROLL) {
+        int32_t ix[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) ix[u] = idx[((g + u) << 6) + lane];
+        uint32_t v[UNROLL][L::m * L::W];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) trove::detail::tile_gather_indexed<64>(src, ix[u], lane, 0, v[u]);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) trove::detail::tile_store_words<64>(dst + ((g + u) << 6), lane, v[u]);
+    }
+    for (; g < full_end; ++g) {
+        uint32_t v[L::m * L::W];
+        trove::detail::tile_gather_indexed<64>(src, idx[(g << 6) + lane], lane, 0, v);
+        trove::detail::tile_store_words<64>(dst + (g << 6), lane, v);
+    }
+    if (g1 > full && (n & 63)) {
+        const int64_t i = (full << 6) + lane;
+        if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + idx[i]), dst + i);
+    }
+}
+
+// coalesced_ptr path (tests/benchmark.cu:90-99): one group per iteration; a
+// partial last group runs the tile cascade with its missing lanes inactive.
+template <typename T>
+__device__ __forceinline__ void gather_range_shfl(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                  T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1) {
+    const int lane = trove::lane_id();
+    trove::coalesced_ptr<T> s(const_cast<T*>(src));
+    trove::coalesced_ptr<T> r(dst);
+    for (int64_t g = g0; g < g1; ++g) {
+        const int64_t i = (g << 6) + lane;
+        if (i < n) {
+            const T v = s[idx[i]];
+            r[i] = v;
+        }
+    }
+}
+
+// Naive per-lane gather (tests/benchmark.cu:112-119).
+template <typename T>
+__device__ __forceinline__ void gather_range_direct(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                    T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1) {
+    const int lane = trove::lane_id();
+    for (int64_t g = g0; g < g1; ++g) {
+        const int64_t i = (g << 6) + lane;
+        if (i < n) dst[i] = src[idx[i]];
+    }
+}
+
+}  // namespace trove_amd

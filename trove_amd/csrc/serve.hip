@@ -1,0 +1,458 @@
+// Continuously batched gather server over a device-resident struct table.
+//
+// No reference counterpart: the reference is a device library. This engine
+// serves requests "gather these rows of the table" with:
+//  * admission by an HBM budget: every admitted request owns a contiguous
+//    range of rows in a device arena (its output rows and, at the same row
+//    offsets in a second arena, its indices); requests wait in FIFO order
+//    until a request slot and a range that fits are free;
+//  * continuous batching: each step gathers the next slice (at most
+//    `slice_indices`) of every active request in ONE launch of the
+//    persistent batch-gather kernel, so requests admitted while others are
+//    still on the GPU join the next step and their work overlaps on the
+//    device;
+//  * pipelining: up to kInflight steps are queued on the compute stream;
+//    host index uploads run on a separate copy stream and the compute stream
+//    waits on them by event;
+//  * completion: a request is done when the step holding its last slice has
+//    finished (event query); its output stays in the arena until released.
+// The host side is single-threaded: the caller drives it with step() (or
+// replay()), and every call is non-blocking except drain().
+#include "batch.h"
+
+#include <chrono>
+#include <deque>
+#include <map>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+namespace trove_amd {
+
+namespace {
+
+constexpr int kInflight = 4;
+
+double now_s() {
+    using clk = std::chrono::steady_clock;
+    return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+// First-fit allocator of row ranges in [0, capacity).
+class range_allocator {
+  public:
+    explicit range_allocator(int64_t capacity) { free_[0] = capacity; }
+    int64_t alloc(int64_t len) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= len) {
+                const int64_t start = it->first, rest = it->second - len;
+                free_.erase(it);
+                if (rest > 0) free_[start + len] = rest;
+                return start;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t start, int64_t len) {
+        auto next = free_.lower_bound(start);
+        if (next != free_.end() && start + len == next->first) {
+            len += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == start) {
+                prev->second += len;
+                return;
+            }
+        }
+        free_[start] = len;
+    }
+
+  private:
+    std::map<int64_t, int64_t> free_;
+};
+
+enum req_state { QUEUED = 0, ACTIVE = 1, DONE = 2 };
+
+struct request {
+    uint64_t id = 0;
+    const int32_t* idx = nullptr;  // caller's indices (host or device)
+    bool idx_on_device = false;
+    int64_t n = 0;
+    int64_t rows = 0;       // arena rows reserved (>= 1)
+    int64_t row0 = -1;      // arena offset
+    int64_t scheduled = 0;  // indices handed to steps so far
+    int state = QUEUED;
+    double t_submit = 0, t_done = 0;
+    hipEvent_t upload = nullptr;  // copy-stream event for host indices
+};
+
+struct step_rec {
+    hipEvent_t done = nullptr;
+    std::vector<uint64_t> finishing;
+    bool busy = false;
+};
+
+}  // namespace
+
+}  // namespace trove_amd
+
+using namespace trove_amd;
+
+struct trove_serve {
+    const void* table = nullptr;
+    int64_t n_rows = 0;
+    int32_t row_bytes = 0;
+    int64_t capacity_rows = 0;
+    int32_t max_slots = 0;
+    int64_t slice = 0;
+    int32_t variant = TROVE_VARIANT_TILE;
+    hipStream_t stream = nullptr;       // compute (caller's stream)
+    hipStream_t copy_stream = nullptr;  // index uploads
+    uint8_t* out_arena = nullptr;
+    int32_t* idx_arena = nullptr;
+    range_allocator* arena = nullptr;
+    std::deque<uint64_t> queue;   // QUEUED, FIFO
+    std::vector<uint64_t> active;  // ACTIVE, admission order
+    std::unordered_map<uint64_t, request> reqs;
+    uint64_t next_id = 1;
+    // step ring
+    step_rec steps[kInflight];
+    job_desc* h_jobs[kInflight] = {};  // pinned
+    job_desc* d_jobs[kInflight] = {};
+    int max_jobs = 0;
+    uint64_t step_seq = 0;  // next step slot = step_seq % kInflight
+    uint64_t retired_seq = 0;
+    std::vector<uint64_t> newly_done;  // completions not yet seen by replay
+    // statistics
+    int64_t completed = 0, indices_done = 0, steps_launched = 0;
+};
+
+static void free_serve(trove_serve* s) {
+    for (int k = 0; k < kInflight; ++k) {
+        if (s->steps[k].done) (void)hipEventDestroy(s->steps[k].done);
+        if (s->h_jobs[k]) (void)hipHostFree(s->h_jobs[k]);
+        if (s->d_jobs[k]) (void)hipFree(s->d_jobs[k]);
+    }
+    for (auto& kv : s->reqs)
+        if (kv.second.upload) (void)hipEventDestroy(kv.second.upload);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+    if (s->out_arena) (void)hipFree(s->out_arena);
+    if (s->idx_arena) (void)hipFree(s->idx_arena);
+    delete s->arena;
+    delete s;
+}
+
+// Marks the requests of every finished step done, oldest first.
+static int retire(trove_serve* s) {
+    while (s->retired_seq < s->step_seq) {
+        step_rec& st = s->steps[s->retired_seq % kInflight];
+        const hipError_t q = hipEventQuery(st.done);
+        if (q == hipErrorNotReady) break;
+        if (q != hipSuccess) return hip_status(q);
+        const double t = now_s();
+        for (uint64_t id : st.finishing) {
+            request& r = s->reqs[id];
+            r.state = DONE;
+            r.t_done = t;
+            s->completed++;
+            s->indices_done += r.n;
+            s->newly_done.push_back(id);
+        }
+        st.finishing.clear();
+        st.busy = false;
+        s->retired_seq++;
+    }
+    return TROVE_OK;
+}
+
+// Admits queued requests in FIFO order while a slot and an arena range fit.
+static int admit(trove_serve* s) {
+    while (!s->queue.empty() && (int)s->active.size() < s->max_slots) {
+        request& r = s->reqs[s->queue.front()];
+        const int64_t row0 = s->arena->alloc(r.rows);
+        if (row0 < 0) break;
+        r.row0 = row0;
+        if (r.n > 0) {
+            int32_t* dst = s->idx_arena + row0;
+            const hipMemcpyKind kind = r.idx_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+            int rc = hip_status(hipMemcpyAsync(dst, r.idx, sizeof(int32_t) * r.n, kind, s->copy_stream));
+            if (rc == TROVE_OK && !r.upload) rc = hip_status(hipEventCreateWithFlags(&r.upload, hipEventDisableTiming));
+            if (rc == TROVE_OK) rc = hip_status(hipEventRecord(r.upload, s->copy_stream));
+            if (rc != TROVE_OK) return rc;
+        }
+        r.state = ACTIVE;
+        s->active.push_back(r.id);
+        s->queue.pop_front();
+    }
+    return TROVE_OK;
+}
+
+// Builds and launches one step: the next slice of every active request.
+static int launch_step(trove_serve* s) {
+    if (s->step_seq - s->retired_seq >= (uint64_t)kInflight) return TROVE_OK;  // ring full
+    const int k = (int)(s->step_seq % kInflight);
+    step_rec& st = s->steps[k];
+    job_desc* jobs = s->h_jobs[k];
+    int nj = 0;
+    int64_t items = 0;
+    std::vector<uint64_t> still_active;
+    still_active.reserve(s->active.size());
+    for (uint64_t id : s->active) {
+        request& r = s->reqs[id];
+        const int64_t left = r.n - r.scheduled;
+        if (nj >= s->max_jobs) {
+            still_active.push_back(id);
+            continue;
+        }
+        const int64_t cnt = left < s->slice ? left : s->slice;
+        if (r.upload && r.scheduled == 0) {
+            const int rc = hip_status(hipStreamWaitEvent(s->stream, r.upload, 0));
+            if (rc != TROVE_OK) return rc;
+        }
+        job_desc& d = jobs[nj++];
+        d.src = s->table;
+        d.idx = s->idx_arena + r.row0 + r.scheduled;
+        d.dst = s->out_arena + (r.row0 + r.scheduled) * (int64_t)s->row_bytes;
+        d.n = cnt;
+        d.bytes = s->row_bytes;
+        d.variant = s->variant;
+        d.chunk_groups = 8;
+        d.pad = 0;
+        d.first_item = items;
+        items += ((cnt + 63) / 64 + d.chunk_groups - 1) / d.chunk_groups;
+        r.scheduled += cnt;
+        if (r.scheduled == r.n)
+            st.finishing.push_back(id);
+        else
+            still_active.push_back(id);
+    }
+    if (nj == 0) return TROVE_OK;
+    s->active.swap(still_active);
+    int rc = hip_status(
+        hipMemcpyAsync(s->d_jobs[k], jobs, sizeof(job_desc) * nj, hipMemcpyHostToDevice, s->stream));
+    if (rc == TROVE_OK) rc = launch_job_table(s->d_jobs[k], nj, items, s->stream);
+    if (rc == TROVE_OK) rc = hip_status(hipEventRecord(st.done, s->stream));
+    if (rc != TROVE_OK) return rc;
+    st.busy = true;
+    s->step_seq++;
+    s->steps_launched++;
+    return TROVE_OK;
+}
+
+static int serve_step(trove_serve* s) {
+    int rc = retire(s);
+    if (rc == TROVE_OK) rc = admit(s);
+    if (rc == TROVE_OK) rc = launch_step(s);
+    return rc;
+}
+
+// Zero-length requests are done at submission.
+static void finish_empty(trove_serve* s, request& r) {
+    r.state = DONE;
+    r.t_done = now_s();
+    s->completed++;
+}
+
+extern "C" {
+
+int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int64_t budget_bytes,
+                       int32_t max_slots, int64_t slice_indices, int32_t variant, void* stream,
+                       trove_serve** out) {
+    if (!out) return TROVE_ERR_INVALID;
+    *out = nullptr;
+    if (!table || n_rows <= 0 || n_rows > INT32_MAX) return TROVE_ERR_INVALID;
+    if (row_bytes < 4 || row_bytes > 64 || row_bytes % 4 != 0) return TROVE_ERR_SIZE;
+    if (budget_bytes < row_bytes || max_slots <= 0 || slice_indices < 64) return TROVE_ERR_INVALID;
+    if (variant != TROVE_VARIANT_AUTO && variant != TROVE_VARIANT_TILE && variant != TROVE_VARIANT_SHFL &&
+        variant != TROVE_VARIANT_DIRECT)
+        return TROVE_ERR_VARIANT;
+    if (!aligned(table, word_bytes_of(row_bytes))) return TROVE_ERR_ALIGN;
+    trove_serve* s = new (std::nothrow) trove_serve();
+    if (!s) return TROVE_ERR_NOMEM;
+    s->table = table;
+    s->n_rows = n_rows;
+    s->row_bytes = row_bytes;
+    s->capacity_rows = budget_bytes / row_bytes;
+    s->max_slots = max_slots;
+    s->slice = slice_indices;
+    s->variant = variant == TROVE_VARIANT_AUTO ? TROVE_VARIANT_TILE : variant;
+    s->stream = static_cast<hipStream_t>(stream);
+    s->max_jobs = max_slots;
+    s->arena = new (std::nothrow) range_allocator(s->capacity_rows);
+    int rc = s->arena ? TROVE_OK : TROVE_ERR_NOMEM;
+    if (rc == TROVE_OK) rc = hip_status(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->out_arena, (size_t)(s->capacity_rows * row_bytes)));
+    if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->idx_arena, (size_t)(s->capacity_rows * sizeof(int32_t))));
+    for (int k = 0; k < kInflight && rc == TROVE_OK; ++k) {
+        rc = hip_status(hipEventCreateWithFlags(&s->steps[k].done, hipEventDisableTiming));
+        if (rc == TROVE_OK) rc = hip_status(hipHostMalloc((void**)&s->h_jobs[k], sizeof(job_desc) * s->max_jobs));
+        if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->d_jobs[k], sizeof(job_desc) * s->max_jobs));
+    }
+    if (rc != TROVE_OK) {
+        free_serve(s);
+        return rc == TROVE_ERR_HIP && trove_last_hip_error() == (int)hipErrorOutOfMemory ? TROVE_ERR_NOMEM : rc;
+    }
+    *out = s;
+    return TROVE_OK;
+}
+
+int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on_device, uint64_t* request_id) {
+    if (!s || !request_id || n < 0 || (n > 0 && !idx)) return TROVE_ERR_INVALID;
+    if (n > s->capacity_rows) return TROVE_ERR_BUDGET;
+    if (!idx_on_device) {
+        for (int64_t i = 0; i < n; ++i)
+            if (idx[i] < 0 || idx[i] >= s->n_rows) return TROVE_ERR_RANGE;
+    } else if (!aligned(idx, 4)) {
+        return TROVE_ERR_ALIGN;
+    }
+    request r;
+    r.id = s->next_id++;
+    r.idx = idx;
+    r.idx_on_device = idx_on_device != 0;
+    r.n = n;
+    r.rows = n > 0 ? n : 1;
+    r.t_submit = now_s();
+    *request_id = r.id;
+    auto& slot = s->reqs[r.id];
+    slot = r;
+    if (n == 0) {
+        finish_empty(s, slot);
+        slot.row0 = -1;
+        return TROVE_OK;
+    }
+    s->queue.push_back(r.id);
+    return TROVE_OK;
+}
+
+int trove_serve_step(trove_serve* s, int32_t* in_flight) {
+    if (!s) return TROVE_ERR_INVALID;
+    const int rc = serve_step(s);
+    if (in_flight) *in_flight = (int32_t)(s->queue.size() + s->active.size() + (s->step_seq - s->retired_seq));
+    return rc;
+}
+
+int trove_serve_poll(trove_serve* s, uint64_t request_id, int32_t* done, double* latency_s) {
+    if (!s || !done) return TROVE_ERR_INVALID;
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
+    *done = it->second.state == DONE;
+    if (latency_s) *latency_s = it->second.state == DONE ? it->second.t_done - it->second.t_submit : -1.0;
+    return TROVE_OK;
+}
+
+int trove_serve_output(trove_serve* s, uint64_t request_id, void** dev_out) {
+    if (!s || !dev_out) return TROVE_ERR_INVALID;
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
+    if (it->second.state != DONE) return TROVE_ERR_AGAIN;
+    *dev_out = it->second.row0 >= 0 ? s->out_arena + it->second.row0 * (int64_t)s->row_bytes : nullptr;
+    return TROVE_OK;
+}
+
+int trove_serve_fetch(trove_serve* s, uint64_t request_id, void* out) {
+    if (!s || !out) return TROVE_ERR_INVALID;
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
+    if (it->second.state != DONE) return TROVE_ERR_AGAIN;
+    if (it->second.n == 0) return TROVE_OK;
+    const void* src = s->out_arena + it->second.row0 * (int64_t)s->row_bytes;
+    return hip_status(hipMemcpy(out, src, (size_t)(it->second.n * s->row_bytes), hipMemcpyDefault));
+}
+
+int trove_serve_release(trove_serve* s, uint64_t request_id) {
+    if (!s) return TROVE_ERR_INVALID;
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
+    if (it->second.state != DONE) return TROVE_ERR_AGAIN;
+    if (it->second.row0 >= 0) s->arena->release(it->second.row0, it->second.rows);
+    if (it->second.upload) (void)hipEventDestroy(it->second.upload);
+    s->reqs.erase(it);
+    return TROVE_OK;
+}
+
+int trove_serve_drain(trove_serve* s) {
+    if (!s) return TROVE_ERR_INVALID;
+    for (;;) {
+        int rc = serve_step(s);
+        if (rc != TROVE_OK) return rc;
+        if (s->queue.empty() && s->active.empty() && s->retired_seq == s->step_seq) return TROVE_OK;
+        if (!s->queue.empty() && s->active.empty() && s->retired_seq == s->step_seq) {
+            // Nothing running and the head does not fit: only releases free space.
+            return TROVE_ERR_FULL;
+        }
+        if (s->retired_seq < s->step_seq) {
+            rc = hip_status(hipEventSynchronize(s->steps[s->retired_seq % kInflight].done));
+            if (rc != TROVE_OK) return rc;
+        }
+    }
+}
+
+int trove_serve_stats(const trove_serve* s, int64_t* completed, int64_t* indices_done, int64_t* steps,
+                      int32_t* queued, int32_t* active) {
+    if (!s) return TROVE_ERR_INVALID;
+    if (completed) *completed = s->completed;
+    if (indices_done) *indices_done = s->indices_done;
+    if (steps) *steps = s->steps_launched;
+    if (queued) *queued = (int32_t)s->queue.size();
+    if (active) *active = (int32_t)s->active.size();
+    return TROVE_OK;
+}
+
+int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
+                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s) {
+    if (!s || n_req < 0 || (n_req > 0 && (!arrival_s || !idx || !n || !latency_s))) return TROVE_ERR_INVALID;
+    std::vector<uint64_t> ids(n_req, 0);
+    std::unordered_map<uint64_t, int32_t> slot_of;
+    slot_of.reserve((size_t)n_req * 2);
+    int32_t next = 0, done = 0;
+    s->newly_done.clear();
+    const double t0 = now_s();
+    while (done < n_req) {
+        const double t = now_s() - t0;
+        while (next < n_req && arrival_s[next] <= t) {
+            const int rc = trove_serve_submit(s, idx[next], n[next], idx_on_device, &ids[next]);
+            if (rc != TROVE_OK) return rc;
+            // Latency counts from the scheduled arrival, not from submission.
+            request& r = s->reqs[ids[next]];
+            r.t_submit = t0 + arrival_s[next];
+            if (r.state == DONE) {
+                latency_s[next] = r.t_done - r.t_submit;
+                ++done;
+                if (release_on_done) trove_serve_release(s, ids[next]);
+            } else {
+                slot_of[ids[next]] = next;
+            }
+            ++next;
+        }
+        const int rc = serve_step(s);
+        if (rc != TROVE_OK) return rc;
+        if (!s->queue.empty() && s->active.empty() && s->retired_seq == s->step_seq && s->newly_done.empty()) {
+            // Nothing runs and the queue head does not fit: only a release can
+            // free space, and outputs are being kept.
+            return TROVE_ERR_FULL;
+        }
+        for (uint64_t id : s->newly_done) {
+            auto it = slot_of.find(id);
+            if (it == slot_of.end()) continue;
+            const request& r = s->reqs[id];
+            latency_s[it->second] = r.t_done - r.t_submit;
+            ++done;
+            slot_of.erase(it);
+            if (release_on_done) trove_serve_release(s, id);
+        }
+        s->newly_done.clear();
+    }
+    return TROVE_OK;
+}
+
+int trove_serve_destroy(trove_serve* s) {
+    if (!s) return TROVE_ERR_INVALID;
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->copy_stream);
+    free_serve(s);
+    return TROVE_OK;
+}
+
+}  // extern "C"
