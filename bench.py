@@ -44,6 +44,11 @@ def parse_args():
     p.add_argument("--cpu-n-log2", type=int, default=23, help="structs per size in the CPU sample (log2)")
     p.add_argument("--no-naive", action="store_true", help="skip the naive-kernel comparison")
     p.add_argument("--seed", type=int, default=20260101)
+    p.add_argument("--config", choices=("all", "sweep", "batched", "served"), default="all",
+                   help="all: the sweep line with batched and served summaries attached")
+    p.add_argument("--jobs", type=int, default=1024, help="batched: number of gather jobs")
+    p.add_argument("--tune-table", type=str, default=None, help="batched: tuner table to load instead of tuning")
+    p.add_argument("--requests", type=int, default=8192, help="served: number of requests")
     return p.parse_args()
 
 
@@ -119,6 +124,192 @@ def cpu_baseline(ks, n_log2, seed):
     }
 
 
+def _threaded(fn, items, workers=16):
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(fn, items))
+
+
+def timed_runs(torch, fn, reps):
+    out = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        out.append(e0.elapsed_time(e1))
+    return out
+
+
+def bench_batched(args, torch, T):
+    """1024 independent gather jobs: struct size uniform over 4..64 B, length
+    log-uniform 2^10..2^22 structs, index pattern uniform over identity /
+    uniform random permutation / sorted runs (geometric, mean 32); source
+    length = job length. Persistent launch vs its hipGraph replay vs one
+    launch per job, at equal output (bit-exact compared)."""
+    import numpy as np
+
+    from trove_amd import tune
+
+    rng = np.random.default_rng(args.seed + 3)
+    nj = args.jobs
+    sizes = 4 * rng.integers(1, 17, size=nj)
+    lengths = T.gen_loguniform(nj, 1 << 10, 1 << 22, args.seed + 4)
+    patterns = [tune.PATTERNS[p] for p in rng.integers(0, 3, size=nj)]
+    align = lambda x: (x + 255) // 256 * 256
+    src_off, idx_off, tot_src, tot_idx = [], [], 0, 0
+    for s, L in zip(sizes, lengths):
+        src_off.append(tot_src)
+        tot_src += align(int(s) * int(L))
+        idx_off.append(tot_idx)
+        tot_idx += align(4 * int(L)) // 4
+    dev = torch.device("cuda:0")
+    src = torch.empty(tot_src // 4, dtype=torch.int32, device=dev)
+    dst_a = torch.empty_like(src)
+    dst_b = torch.empty_like(src)
+    T.fill_words_device(src, src.numel(), args.seed + 5)
+    t0 = time.time()
+    host_idx = np.empty(tot_idx, dtype=np.int32)
+
+    def gen(j):
+        L = int(lengths[j])
+        host_idx[idx_off[j]:idx_off[j] + L] = T.gen_indices(L, L, patterns[j], args.seed * 7919 + j)
+
+    _threaded(gen, range(nj))
+    idx = torch.from_numpy(host_idx).to(dev)
+    log(f"bench batched: {nj} jobs, {int(lengths.sum())} structs, inputs in {time.time() - t0:.1f}s")
+    if args.tune_table and os.path.exists(args.tune_table):
+        table = tune.load(args.tune_table)
+    else:
+        t0 = time.time()
+        table = tune.autotune(reps=3, seed=args.seed)
+        log(f"bench batched: tuned {len(table['entries'])} shapes in {time.time() - t0:.1f}s")
+    base = src.data_ptr()
+    jobs_a, jobs_b, choice = [], [], {}
+    for j in range(nj):
+        s, L = int(sizes[j]), int(lengths[j])
+        v, c = tune.choose(table, s, patterns[j], L)
+        choice[v] = choice.get(v, 0) + 1
+        sp, ip = base + src_off[j], idx.data_ptr() + 4 * idx_off[j]
+        jobs_a.append((sp, ip, dst_a.data_ptr() + src_off[j], L, s, v, c))
+        jobs_b.append((sp, ip, dst_b.data_ptr() + src_off[j], L, s, v, c))
+    ba, bb = T.Batch(jobs_a), T.Batch(jobs_b)
+    items, nbytes = ba.stats()
+    ba.run("persistent")
+    bb.run("looped")
+    torch.cuda.synchronize()
+    same = all(torch.equal(dst_a[i:i + (1 << 26)], dst_b[i:i + (1 << 26)]) for i in range(0, dst_a.numel(), 1 << 26))
+    reps = 5
+    t_pers = statistics.median(timed_runs(torch, lambda: ba.run("persistent"), reps))
+    ba.run("graph")
+    t_graph = statistics.median(timed_runs(torch, lambda: ba.run("graph"), reps))
+    t_loop = statistics.median(timed_runs(torch, lambda: bb.run("looped"), reps))
+    # Spot-check a few jobs against the oracle on the host.
+    import oracle
+
+    checked = 0
+    for j in list(range(0, nj, max(1, nj // 8)))[:8]:
+        s, L = int(sizes[j]), int(lengths[j])
+        src_h = src[src_off[j] // 4:(src_off[j] + s * L) // 4].cpu().numpy().view(np.uint8)
+        want = oracle.gather(src_h, host_idx[idx_off[j]:idx_off[j] + L], s)
+        got = dst_a[src_off[j] // 4:(src_off[j] + s * L) // 4].cpu().numpy().view(np.uint8)
+        same = same and np.array_equal(got, want)
+        checked += 1
+    ba.close()
+    bb.close()
+    return {
+        "workload": (f"{nj} gather jobs, struct 4..64 B uniform, length log-uniform 2^10..2^22, pattern uniform "
+                     "{identity, random permutation, sorted runs mean 32}"),
+        "jobs": nj, "structs": int(lengths.sum()), "work_items": items, "algorithmic_bytes": nbytes,
+        "persistent_ms": round(t_pers, 3), "graph_ms": round(t_graph, 3), "looped_ms": round(t_loop, 3),
+        "persistent_jobs_per_s": round(nj / (t_pers * 1e-3), 1),
+        "graph_jobs_per_s": round(nj / (t_graph * 1e-3), 1),
+        "looped_jobs_per_s": round(nj / (t_loop * 1e-3), 1),
+        "persistent_gbs": round(nbytes / (t_pers * 1e-3) / 1e9, 1),
+        "graph_gbs": round(nbytes / (t_graph * 1e-3) / 1e9, 1),
+        "speedup_persistent_vs_looped": round(t_loop / t_pers, 3),
+        "speedup_graph_vs_looped": round(t_loop / t_graph, 3),
+        "variants_chosen": choice,
+        "bit_exact": bool(same), "oracle_checked_jobs": checked,
+    }
+
+
+def bench_served(args, torch, T):
+    """Served gather: table of 2^24 rows x 32 B; requests of log-uniform
+    2^8..2^18 indices drawn Zipf(1.1) over the table, indices device-resident;
+    a saturation run (all requests queued at t = 0) fixes the request rate,
+    then Poisson arrivals at 50% and 80% of it; 4 GiB output budget."""
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    n_rows, rb = 1 << 24, 32
+    table = torch.empty(n_rows * rb // 4, dtype=torch.int32, device=dev)
+    T.fill_words_device(table, table.numel(), args.seed + 11)
+    R = args.requests
+    sizes = T.gen_loguniform(R, 1 << 8, 1 << 18, args.seed + 12)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    host_idx = np.empty(int(sizes.sum()), dtype=np.int32)
+    t0 = time.time()
+
+    def gen(i):
+        host_idx[offs[i]:offs[i] + sizes[i]] = T.gen_zipf(int(sizes[i]), n_rows, 1.1, args.seed * 31 + i)
+
+    _threaded(gen, range(R))
+    idx = torch.from_numpy(host_idx).to(dev)
+    torch.cuda.synchronize()
+    log(f"bench served: {R} requests, {host_idx.size} indices, inputs in {time.time() - t0:.1f}s")
+    ptrs = idx.data_ptr() + 4 * offs
+    budget = 4 << 30
+    total_bytes = 68 * int(sizes.sum())
+
+    def run(arrivals):
+        k = len(arrivals)
+        srv = T.Server(table, n_rows, rb, budget_bytes=budget, max_slots=1024, slice_indices=1 << 16,
+                       variant="tile")
+        t_start = time.perf_counter()
+        lat = srv.replay(arrivals, ptrs[:k], sizes[:k], on_device=True, release_on_done=True)
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.close()
+        return lat, wall, st
+
+    run(np.zeros(min(R, 256)))  # warm-up
+    lat, wall, st = run(np.zeros(R))
+    sat_rate = R / wall
+    out = {
+        "workload": "table 2^24 x 32 B; requests log-uniform 2^8..2^18 indices, Zipf(1.1) rows; 4 GiB budget",
+        "requests": R, "indices": int(sizes.sum()),
+        "saturation": {"req_per_s": round(sat_rate, 1), "gbs": round(total_bytes / wall / 1e9, 1),
+                       "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"]},
+    }
+    rng = np.random.default_rng(args.seed + 13)
+    for frac in (0.5, 0.8):
+        rate = frac * sat_rate
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=R))
+        lat, wall, st = run(arrivals)
+        out[f"poisson_{int(frac * 100)}pct"] = {
+            "rate_req_per_s": round(rate, 1), "gbs": round(total_bytes / wall / 1e9, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"]}
+    # Bit-exactness: a few requests alone vs the oracle.
+    import oracle
+
+    host_table = table.cpu().numpy().view(np.uint8)
+    ok = True
+    for i in range(0, R, max(1, R // 6))[:6]:
+        srv = T.Server(table, n_rows, rb, budget_bytes=budget, max_slots=4, slice_indices=1 << 16)
+        r = host_idx[offs[i]:offs[i] + sizes[i]]
+        rid = srv.submit(r, r.size, on_device=False)
+        srv.drain()
+        ok = ok and np.array_equal(srv.fetch(rid, r.size), oracle.gather(host_table, r, rb))
+        srv.close()
+    out["bit_exact_sampled"] = bool(ok)
+    return out
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -133,6 +324,13 @@ def main():
 
     T.lib()  # fail loudly if the HIP library is missing
     assert torch.cuda.is_available(), "no GPU visible"
+    if args.config == "batched":
+        print(json.dumps({"metric": "batched gather jobs/s", "batched": bench_batched(args, torch, T)}), flush=True)
+        return 0
+    if args.config == "served":
+        print(json.dumps({"metric": "served gather p99 latency", "served": bench_served(args, torch, T)}),
+              flush=True)
+        return 0
     dev = torch.device("cuda:0")
     ks = parse_k(args.k)
     n = 1 << args.n_log2
@@ -228,13 +426,24 @@ def main():
         if dname in tr:
             traffic = tr[dname]
 
+    del src, dst, idx
+    torch.cuda.empty_cache()
+    extra = {}
+    if args.config == "all":
+        log("bench: batched jobs")
+        extra["batched"] = bench_batched(args, torch, T)
+        torch.cuda.empty_cache()
+        log("bench: served gather")
+        extra["served"] = bench_served(args, torch, T)
+        torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
         cpu = cpu_baseline(ks, args.cpu_n_log2, args.seed)
 
     out = {
-        "metric": "AoS copy/gather GB/s (% HBM peak) vs naive loads",
+        "metric": "AoS copy/gather GB/s (% HBM peak) vs naive loads; batched jobs/s, p99 latency",
         "value": round(value, 1),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -267,6 +476,7 @@ def main():
         "cpu_baseline": cpu,
         "memcpy_ceiling_gbs": naive.get("memcpy_ceiling", {}).get("gbs"),
         "sweep": sweep,
+        **extra,
     }
     print(json.dumps(out), flush=True)
     return 0

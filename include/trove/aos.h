@@ -62,19 +62,19 @@ struct word_layout {
 };
 
 // Coalesced loads of a tile's words: word j of rank id is base[id + j*n].
-template <int n, typename T>
+template <int n, typename T, typename Policy = cached>
 __device__ __forceinline__ void tile_load_words(const T* tile_base, int id,
                                                 uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
     typedef word_layout<T> L;
     const typename L::U* base = reinterpret_cast<const typename L::U*>(tile_base);
 #pragma unroll
     for (int j = 0; j < L::m; ++j) {
-        const typename L::U u = base[id + j * n];
+        const typename L::U u = load_word<Policy>(base + id + j * n);
         __builtin_memcpy(&v[j * L::W], &u, sizeof(u));
     }
 }
 
-template <int n, typename T>
+template <int n, typename T, typename Policy = cached>
 __device__ __forceinline__ void tile_store_words(T* tile_base, int id,
                                                  const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
     typedef word_layout<T> L;
@@ -83,7 +83,7 @@ __device__ __forceinline__ void tile_store_words(T* tile_base, int id,
     for (int j = 0; j < L::m; ++j) {
         typename L::U u;
         __builtin_memcpy(&u, &v[j * L::W], sizeof(u));
-        base[id + j * n] = u;
+        store_word<Policy>(base + id + j * n, u);
     }
 }
 
@@ -208,26 +208,39 @@ __device__ __forceinline__ void tile_scatter_indexed(T* base, int index, int id,
 
 }  // namespace detail
 
-template <std::size_t tile_size = WARP_SIZE, typename T>
+// Policy is trove::cached (default: ordinary accesses, as in the reference)
+// or trove::streaming (nontemporal accesses for data touched once).
+template <std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T>
 __device__ typename enable_if<detail::use_shfl<T, tile_size>::value, T>::type load_warp_contiguous(const T* src) {
     constexpr int n = (int)tile_size;
     typedef detail::word_layout<T> L;
     const int lane = lane_id();
     const int id = lane & (n - 1);
     uint32_t v[L::m * L::W];
-    detail::tile_load_words<n>(src - id, id, v);
+    detail::tile_load_words<n, T, Policy>(src - id, id, v);
     detail::r2c_words<n, L::m, L::W>(v, id, lane & ~(n - 1));
     T out;
     __builtin_memcpy(&out, v, sizeof(T));
     return out;
 }
 
-template <std::size_t tile_size = WARP_SIZE, typename T>
+template <std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T>
 __device__ typename enable_if<detail::use_direct<T, tile_size>::value, T>::type load_warp_contiguous(const T* src) {
-    return detail::divergent_load(src);
+    if constexpr (detail::use_divergent<T>::value) {
+        typedef detail::word_layout<T> L;
+        const typename L::U* p = reinterpret_cast<const typename L::U*>(src);
+        typename L::U w[L::m];
+#pragma unroll
+        for (int j = 0; j < L::m; ++j) w[j] = detail::load_word<Policy>(p + j);
+        T out;
+        __builtin_memcpy(&out, w, sizeof(T));
+        return out;
+    } else {
+        return detail::divergent_load(src);
+    }
 }
 
-template <std::size_t tile_size = WARP_SIZE, typename T>
+template <std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T>
 __device__ typename enable_if<detail::use_shfl<T, tile_size>::value>::type store_warp_contiguous(const T& data,
                                                                                                  T* dest) {
     constexpr int n = (int)tile_size;
@@ -237,13 +250,22 @@ __device__ typename enable_if<detail::use_shfl<T, tile_size>::value>::type store
     uint32_t v[L::m * L::W];
     __builtin_memcpy(v, &data, sizeof(T));
     detail::c2r_words<n, L::m, L::W>(v, id, lane & ~(n - 1));
-    detail::tile_store_words<n>(dest - id, id, v);
+    detail::tile_store_words<n, T, Policy>(dest - id, id, v);
 }
 
-template <std::size_t tile_size = WARP_SIZE, typename T>
+template <std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T>
 __device__ typename enable_if<detail::use_direct<T, tile_size>::value>::type store_warp_contiguous(const T& data,
                                                                                                    T* dest) {
-    detail::divergent_store(data, dest);
+    if constexpr (detail::use_divergent<T>::value) {
+        typedef detail::word_layout<T> L;
+        typename L::U w[L::m];
+        __builtin_memcpy(w, &data, sizeof(T));
+        typename L::U* p = reinterpret_cast<typename L::U*>(dest);
+#pragma unroll
+        for (int j = 0; j < L::m; ++j) detail::store_word<Policy>(p + j, w[j]);
+    } else {
+        detail::divergent_store(data, dest);
+    }
 }
 
 namespace detail {

@@ -364,6 +364,8 @@ class Server:
         import numpy as np
 
         n = len(counts)
+        if len(arrival_s) != n or len(idx_ptrs) != n:
+            raise ValueError("arrival_s, idx_ptrs and counts must have the same length")
         arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
         ptrs = np.ascontiguousarray(idx_ptrs, dtype=np.uint64)
         cnt = np.ascontiguousarray(counts, dtype=np.int64)

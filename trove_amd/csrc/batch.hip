@@ -33,43 +33,47 @@ __device__ __forceinline__ void batch_item(const job_desc& J, int64_t local) {
             gather_range_tile<T, gather_groups_in_flight<T>()>(src, J.idx, dst, J.n, g0, g1);
             return;
         }
-        if (J.variant == TROVE_VARIANT_SHFL) {
-            gather_range_shfl<T>(src, J.idx, dst, J.n, g0, g1);
+        if (J.variant == TROVE_VARIANT_SHFL) {  // items of one group (trove_batch_create)
+            gather_group_shfl<T>(src, J.idx, dst, J.n, g0);
             return;
         }
     }
     gather_range_direct<T>(src, J.idx, dst, J.n, g0, g1);
 }
 
+// One work item per wavefront, no loop over items: in a loop, LICM hoists
+// the per-lane tables of every (size, variant) branch and keeps them all
+// live (275 VGPRs). The grid has exactly total_items wavefronts.
 __global__ __launch_bounds__(kBlock) void k_batch_gather(const job_desc* __restrict__ jobs, int n_jobs,
                                                          int64_t total_items) {
-    for (int64_t it = wave_index(); it < total_items; it += wave_count()) {
-        int lo = 0, hi = n_jobs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].first_item <= it) {
-                lo = mid;
-            } else {
-                hi = mid - 1;
-            }
+    const int64_t it = wave_index();
+    if (it >= total_items) return;
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first_item <= it) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
         }
-        const job_desc J = jobs[lo];
-        const int64_t local = it - J.first_item;
-        switch (J.bytes) {
-#define TROVE_BATCH_CASE(B)    \
-    case B:                    \
+    }
+    const job_desc J = jobs[lo];
+    const int64_t local = it - J.first_item;
+    switch (J.bytes) {
+#define TROVE_BATCH_CASE(B)      \
+    case B:                      \
         batch_item<B>(J, local); \
         break;
-            TROVE_BATCH_SIZES(TROVE_BATCH_CASE)
+        TROVE_BATCH_SIZES(TROVE_BATCH_CASE)
 #undef TROVE_BATCH_CASE
-            default:
-                break;
-        }
+        default:
+            break;
     }
 }
 
 int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, hipStream_t s) {
-    const int64_t blocks = grid_for(total_items > 0 ? total_items : 1);
+    // One wavefront per item: never capped.
+    const int64_t blocks = (total_items + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL(k_batch_gather, dim3((unsigned)blocks), dim3(kBlock), 0, s, d_table, n_jobs, total_items);
     return launch_status();
 }
@@ -131,6 +135,9 @@ int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out)
         d.bytes = bytes;
         d.variant = J.variant == TROVE_VARIANT_AUTO ? TROVE_VARIANT_TILE : J.variant;
         d.chunk_groups = J.chunk_groups > 0 ? J.chunk_groups : 16;
+        // coalesced_ptr items are one group: a loop around the tile cascade
+        // would keep every level's tables live.
+        if (d.variant == TROVE_VARIANT_SHFL) d.chunk_groups = 1;
         d.first_item = item;
         const int64_t groups = (J.n + 63) / 64;
         item += (groups + d.chunk_groups - 1) / d.chunk_groups;

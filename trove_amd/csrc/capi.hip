@@ -40,19 +40,11 @@ int device_cu_count() {
     return cached_cus;
 }
 
-// Blocks for `waves` wavefront work units: one wavefront per unit, capped at
-// TROVE_MAX_BLOCKS_PER_CU (default 16) resident-sized blocks per compute
-// unit, beyond which kernels stride over the remaining work.
-int64_t grid_for(int64_t waves) {
-    static const int64_t per_cu = [] {
-        const char* e = std::getenv("TROVE_MAX_BLOCKS_PER_CU");
-        const long v = e ? std::atol(e) : 0;
-        return (int64_t)(v > 0 ? v : 16);
-    }();
-    int64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int64_t cap = per_cu * device_cu_count();
-    if (blocks > cap) blocks = cap;
-    return blocks < 1 ? 1 : blocks;
+int64_t grid_for(int64_t waves, int waves_per_block) {
+    if (waves < 1) waves = 1;
+    const int64_t blocks = (waves + waves_per_block - 1) / waves_per_block;
+    const int64_t max_grid = 0x7fffffff;
+    return blocks > max_grid ? max_grid : blocks;
 }
 
 typedef int (*copy_fn)(const void*, void*, int64_t, int, int, hipStream_t);

@@ -1,38 +1,50 @@
 // Contiguous AoS copy: dst[i] = src[i].
 //  SHFL   - one struct per lane through load_warp_contiguous /
-//           store_warp_contiguous<TILE> (the Trove low-level API).
+//           store_warp_contiguous<TILE> (the Trove low-level API), with the
+//           streaming (nontemporal) cache policy.
 //  DIRECT - the naive per-lane struct copy the compiler generates.
 // Also the reference benchmark's store-only (counting structs) and load-only
 // (per-struct word sum) kernels, tests/benchmark.cu:52-88.
+//
+// Launch shape (measured on MI355X, 2^26 structs): a one-shot grid, one
+// work-group of B threads per wavefront-sized unit, each wavefront moving
+// UNROLL 64-struct groups. Structs of more than one word per lane (the
+// transposition path) stream best at UNROLL = 1 with 64-thread groups; one-
+// word structs need 1 KiB per wavefront (UNROLL = 16 / words) to keep
+// enough bytes in flight.
 #include "common.h"
 
 namespace trove_amd {
 
-// 64-struct groups a wavefront keeps in flight per loop iteration.
 template <typename T>
-constexpr int groups_in_flight() {
-    return sizeof(T) <= 16 ? 8 : sizeof(T) <= 32 ? 4 : sizeof(T) <= 64 ? 2 : 1;
-}
+struct copy_shape {
+    static constexpr int words = sizeof(T) / 4 > 0 ? (int)(sizeof(T) / 4) : 1;
+    static constexpr int m = trove::detail::word_layout<T>::m;
+    static constexpr int unroll = (m == 1 && words < 4) ? 4 / words : 1;
+    static constexpr int block = 64;
+};
 
-template <typename T, int TILE, int UNROLL>
-__global__ __launch_bounds__(kBlock) void k_copy_shfl(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
+template <typename T, int TILE, int UNROLL, int B>
+__global__ __launch_bounds__(B) void k_copy_shfl(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
     const int lane = trove::lane_id();
     const int64_t groups = n >> 6;
-    const int64_t step = wave_count() * UNROLL;
-    int64_t g = wave_index() * UNROLL;
+    const int64_t step = wave_count<B>() * UNROLL;
+    int64_t g = wave_index<B>() * UNROLL;
     for (; g + UNROLL <= groups; g += step) {
         T v[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) v[u] = trove::load_warp_contiguous<TILE>(src + ((g + u) << 6) + lane);
+        for (int u = 0; u < UNROLL; ++u)
+            v[u] = trove::load_warp_contiguous<TILE, trove::streaming>(src + ((g + u) << 6) + lane);
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) trove::store_warp_contiguous<TILE>(v[u], dst + ((g + u) << 6) + lane);
+        for (int u = 0; u < UNROLL; ++u)
+            trove::store_warp_contiguous<TILE, trove::streaming>(v[u], dst + ((g + u) << 6) + lane);
     }
     for (; g < groups; ++g) {
-        const T v = trove::load_warp_contiguous<TILE>(src + (g << 6) + lane);
-        trove::store_warp_contiguous<TILE>(v, dst + (g << 6) + lane);
+        const T v = trove::load_warp_contiguous<TILE, trove::streaming>(src + (g << 6) + lane);
+        trove::store_warp_contiguous<TILE, trove::streaming>(v, dst + (g << 6) + lane);
     }
     // Partial trailing group: the active lanes go through the tile cascade.
-    if (wave_index() == 0 && (n & 63)) {
+    if (wave_index<B>() == 0 && (n & 63)) {
         const int64_t i = (groups << 6) + lane;
         if (i < n) trove::store(trove::load(src + i), dst + i);
     }
@@ -46,10 +58,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_direct(const T* __restrict__ sr
 
 template <typename T, int TILE>
 static int launch_copy_shfl(const T* src, T* dst, int64_t n, hipStream_t s) {
-    constexpr int U = groups_in_flight<T>();
-    const int64_t units = ((n >> 6) + U - 1) / U;
-    const int64_t blocks = grid_for(units > 0 ? units : 1);
-    hipLaunchKernelGGL((k_copy_shfl<T, TILE, U>), dim3((unsigned)blocks), dim3(kBlock), 0, s, src, dst, n);
+    typedef copy_shape<T> C;
+    const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
+    const int64_t blocks = grid_for(units, C::block / 64);
+    hipLaunchKernelGGL((k_copy_shfl<T, TILE, C::unroll, C::block>), dim3((unsigned)blocks), dim3(C::block), 0, s,
+                       src, dst, n);
     return launch_status();
 }
 
@@ -85,24 +98,24 @@ __device__ __forceinline__ T counting_struct(int64_t i) {
     return trove::counting_array<T>::impl((uint32_t)(i * k));
 }
 
-template <typename T, int UNROLL>
-__global__ __launch_bounds__(kBlock) void k_counting_shfl(T* __restrict__ dst, int64_t n) {
+template <typename T, int UNROLL, int B>
+__global__ __launch_bounds__(B) void k_counting_shfl(T* __restrict__ dst, int64_t n) {
     const int lane = trove::lane_id();
     const int64_t groups = n >> 6;
-    const int64_t step = wave_count() * UNROLL;
-    int64_t g = wave_index() * UNROLL;
+    const int64_t step = wave_count<B>() * UNROLL;
+    int64_t g = wave_index<B>() * UNROLL;
     for (; g + UNROLL <= groups; g += step) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const int64_t i = ((g + u) << 6) + lane;
-            trove::store_warp_contiguous(counting_struct<T>(i), dst + i);
+            trove::store_warp_contiguous<trove::WARP_SIZE, trove::streaming>(counting_struct<T>(i), dst + i);
         }
     }
     for (; g < groups; ++g) {
         const int64_t i = (g << 6) + lane;
-        trove::store_warp_contiguous(counting_struct<T>(i), dst + i);
+        trove::store_warp_contiguous<trove::WARP_SIZE, trove::streaming>(counting_struct<T>(i), dst + i);
     }
-    if (wave_index() == 0 && (n & 63)) {
+    if (wave_index<B>() == 0 && (n & 63)) {
         const int64_t i = (groups << 6) + lane;
         if (i < n) trove::store(counting_struct<T>(i), dst + i);
     }
@@ -114,25 +127,26 @@ __global__ __launch_bounds__(kBlock) void k_counting_direct(T* __restrict__ dst,
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = counting_struct<T>(i);
 }
 
-template <typename T, int UNROLL>
-__global__ __launch_bounds__(kBlock) void k_loadsum_shfl(const T* __restrict__ src, uint32_t* __restrict__ out,
-                                                         int64_t n) {
+template <typename T, int UNROLL, int B>
+__global__ __launch_bounds__(B) void k_loadsum_shfl(const T* __restrict__ src, uint32_t* __restrict__ out,
+                                                    int64_t n) {
     const int lane = trove::lane_id();
     const int64_t groups = n >> 6;
-    const int64_t step = wave_count() * UNROLL;
-    int64_t g = wave_index() * UNROLL;
+    const int64_t step = wave_count<B>() * UNROLL;
+    int64_t g = wave_index<B>() * UNROLL;
     for (; g + UNROLL <= groups; g += step) {
         T v[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) v[u] = trove::load_warp_contiguous(src + ((g + u) << 6) + lane);
+        for (int u = 0; u < UNROLL; ++u)
+            v[u] = trove::load_warp_contiguous<trove::WARP_SIZE, trove::streaming>(src + ((g + u) << 6) + lane);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) out[((g + u) << 6) + lane] = trove::sum(v[u]);
     }
     for (; g < groups; ++g) {
         const int64_t i = (g << 6) + lane;
-        out[i] = trove::sum(trove::load_warp_contiguous(src + i));
+        out[i] = trove::sum(trove::load_warp_contiguous<trove::WARP_SIZE, trove::streaming>(src + i));
     }
-    if (wave_index() == 0 && (n & 63)) {
+    if (wave_index<B>() == 0 && (n & 63)) {
         const int64_t i = (groups << 6) + lane;
         if (i < n) out[i] = trove::sum(trove::load(src + i));
     }
@@ -151,12 +165,13 @@ int counting_impl(void* dst_, int64_t n, int variant, hipStream_t s) {
         return TROVE_ERR_SIZE;
     } else {
         typedef blob_t<B> T;
+        typedef copy_shape<T> C;
         T* dst = static_cast<T*>(dst_);
         if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL) {
-            constexpr int U = groups_in_flight<T>();
-            const int64_t units = ((n >> 6) + U - 1) / U;
-            const int64_t blocks = grid_for(units > 0 ? units : 1);
-            hipLaunchKernelGGL((k_counting_shfl<T, U>), dim3((unsigned)blocks), dim3(kBlock), 0, s, dst, n);
+            const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
+            const int64_t blocks = grid_for(units, C::block / 64);
+            hipLaunchKernelGGL((k_counting_shfl<T, C::unroll, C::block>), dim3((unsigned)blocks), dim3(C::block), 0,
+                               s, dst, n);
             return launch_status();
         }
         if (variant == TROVE_VARIANT_DIRECT) {
@@ -174,12 +189,13 @@ int loadsum_impl(const void* src_, uint32_t* out, int64_t n, int variant, hipStr
         return TROVE_ERR_SIZE;
     } else {
         typedef blob_t<B> T;
+        typedef copy_shape<T> C;
         const T* src = static_cast<const T*>(src_);
         if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL) {
-            constexpr int U = groups_in_flight<T>();
-            const int64_t units = ((n >> 6) + U - 1) / U;
-            const int64_t blocks = grid_for(units > 0 ? units : 1);
-            hipLaunchKernelGGL((k_loadsum_shfl<T, U>), dim3((unsigned)blocks), dim3(kBlock), 0, s, src, out, n);
+            const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
+            const int64_t blocks = grid_for(units, C::block / 64);
+            hipLaunchKernelGGL((k_loadsum_shfl<T, C::unroll, C::block>), dim3((unsigned)blocks), dim3(C::block), 0,
+                               s, src, out, n);
             return launch_status();
         }
         if (variant == TROVE_VARIANT_DIRECT) {
@@ -191,9 +207,9 @@ int loadsum_impl(const void* src_, uint32_t* out, int64_t n, int variant, hipStr
     }
 }
 
-#define TROVE_INST_COPY(B)                                                                    \
-    template int copy_impl<B>(const void*, void*, int64_t, int, int, hipStream_t);           \
-    template int counting_impl<B>(void*, int64_t, int, hipStream_t);                         \
+#define TROVE_INST_COPY(B)                                                          \
+    template int copy_impl<B>(const void*, void*, int64_t, int, int, hipStream_t); \
+    template int counting_impl<B>(void*, int64_t, int, hipStream_t);               \
     template int loadsum_impl<B>(const void*, uint32_t*, int64_t, int, hipStream_t);
 TROVE_FOR_EACH_SIZE_IN_SHARD(TROVE_INST_COPY)
 

@@ -7,61 +7,30 @@
 //           elided: the gathered words already sit in the column-side layout
 //           a coalesced tile store takes (and scatter mirrors it).
 //  DIRECT - the naive per-lane kernels (tests/benchmark.cu:112-128).
-#include "common.h"
+#include "gather_body.h"
 
 namespace trove_amd {
 
-template <typename T>
-constexpr int gather_groups_in_flight() {
-    return sizeof(T) <= 16 ? 4 : sizeof(T) <= 32 ? 2 : 1;
+// Each wavefront handles UNROLL 64-struct groups (one-shot grid; a capped
+// grid strides).
+template <typename T, int UNROLL>
+__global__ __launch_bounds__(kBlock) void k_gather_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                        T* __restrict__ dst, int64_t n) {
+    const int64_t groups = (n + 63) >> 6;
+    for (int64_t g0 = wave_index() * UNROLL; g0 < groups; g0 += wave_count() * UNROLL)
+        gather_range_tile<T, UNROLL>(src, idx, dst, n, g0, g0 + UNROLL < groups ? g0 + UNROLL : groups);
 }
 
-// One 64-struct group per wavefront iteration; the trailing partial group
-// runs through the same code with its missing lanes inactive, so the
+// One 64-struct group per wavefront and no loop: inside a loop, LICM would
+// hoist the per-lane schedule tables of all six cascade levels out of it and
+// keep them live together (258 VGPRs at 60-byte structs). The trailing
+// partial group runs the same code with its missing lanes inactive, so the
 // coalesced_ptr cascade picks the widest converged sub-tile for it.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gather_shfl(const T* __restrict__ src, const int32_t* __restrict__ idx,
                                                         T* __restrict__ dst, int64_t n) {
-    const int lane = trove::lane_id();
-    const int64_t groups = (n + 63) >> 6;
-    trove::coalesced_ptr<T> s(const_cast<T*>(src));
-    trove::coalesced_ptr<T> r(dst);
-    for (int64_t g = wave_index(); g < groups; g += wave_count()) {
-        const int64_t i = (g << 6) + lane;
-        if (i < n) {
-            const T v = s[idx[i]];
-            r[i] = v;
-        }
-    }
-}
-
-template <typename T, int UNROLL>
-__global__ __launch_bounds__(kBlock) void k_gather_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
-                                                        T* __restrict__ dst, int64_t n) {
-    typedef trove::detail::word_layout<T> L;
-    const int lane = trove::lane_id();
-    const int64_t groups = n >> 6;
-    const int64_t step = wave_count() * UNROLL;
-    int64_t g = wave_index() * UNROLL;
-    for (; g + UNROLL <= groups; g += step) {
-        int32_t ix[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) ix[u] = idx[((g + u) << 6) + lane];
-        uint32_t v[UNROLL][L::m * L::W];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) trove::detail::tile_gather_indexed<64>(src, ix[u], lane, 0, v[u]);
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) trove::detail::tile_store_words<64>(dst + ((g + u) << 6), lane, v[u]);
-    }
-    for (; g < groups; ++g) {
-        uint32_t v[L::m * L::W];
-        trove::detail::tile_gather_indexed<64>(src, idx[(g << 6) + lane], lane, 0, v);
-        trove::detail::tile_store_words<64>(dst + (g << 6), lane, v);
-    }
-    if (wave_index() == 0 && (n & 63)) {
-        const int64_t i = (groups << 6) + lane;
-        if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + idx[i]), dst + i);
-    }
+    const int64_t g = wave_index();
+    if (g < ((n + 63) >> 6)) gather_group_shfl<T>(src, idx, dst, n, g);
 }
 
 template <typename T>
@@ -74,16 +43,12 @@ __global__ __launch_bounds__(kBlock) void k_gather_direct(const T* __restrict__ 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter_shfl(const T* __restrict__ src, const int32_t* __restrict__ idx,
                                                          T* __restrict__ dst, int64_t n) {
-    const int lane = trove::lane_id();
-    const int64_t groups = (n + 63) >> 6;
-    trove::coalesced_ptr<T> s(const_cast<T*>(src));
-    trove::coalesced_ptr<T> r(dst);
-    for (int64_t g = wave_index(); g < groups; g += wave_count()) {
-        const int64_t i = (g << 6) + lane;
-        if (i < n) {
-            const T v = s[i];
-            r[idx[i]] = v;
-        }
+    const int64_t i = (wave_index() << 6) + trove::lane_id();
+    if (i < n) {
+        trove::coalesced_ptr<T> s(const_cast<T*>(src));
+        trove::coalesced_ptr<T> r(dst);
+        const T v = s[i];
+        r[idx[i]] = v;
     }
 }
 
@@ -100,8 +65,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter_tile(const T* __restrict__ s
         uint32_t v[UNROLL][L::m * L::W];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
-            ix[u] = idx[((g + u) << 6) + lane];
-            trove::detail::tile_load_words<64>(src + ((g + u) << 6), lane, v[u]);
+            ix[u] = trove::detail::load_word<trove::streaming>(idx + ((g + u) << 6) + lane);
+            trove::detail::tile_load_words<64, T, trove::streaming>(src + ((g + u) << 6), lane, v[u]);
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) trove::detail::tile_scatter_indexed<64>(dst, ix[u], lane, 0, v[u]);
@@ -131,7 +96,7 @@ int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int
     T* dst = static_cast<T*>(dst_);
     constexpr bool tiled = trove::detail::use_shfl<T, 64>::value;
     constexpr int U = gather_groups_in_flight<T>();
-    const int64_t units = ((n >> 6) + U - 1) / U;
+    const int64_t units = ((n + 63) / 64 + U - 1) / U;
     if (variant == TROVE_VARIANT_AUTO) variant = tiled ? TROVE_VARIANT_TILE : TROVE_VARIANT_DIRECT;
     if (variant == TROVE_VARIANT_TILE && !tiled) variant = TROVE_VARIANT_DIRECT;
     if (variant == TROVE_VARIANT_SHFL) {

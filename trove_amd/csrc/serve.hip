@@ -1,4 +1,6 @@
 // Continuously batched gather server over a device-resident struct table.
+// (Device-resident request indices are read in place: the caller keeps them
+// valid until the request is done, as for host indices.)
 //
 // No reference counterpart: the reference is a device library. This engine
 // serves requests "gather these rows of the table" with:
@@ -85,7 +87,7 @@ struct request {
     int64_t scheduled = 0;  // indices handed to steps so far
     int state = QUEUED;
     double t_submit = 0, t_done = 0;
-    hipEvent_t upload = nullptr;  // copy-stream event for host indices
+    const int32_t* src_idx = nullptr;  // indices the kernel reads (caller's or arena copy)
 };
 
 struct step_rec {
@@ -113,6 +115,7 @@ struct trove_serve {
     uint8_t* out_arena = nullptr;
     int32_t* idx_arena = nullptr;
     range_allocator* arena = nullptr;
+    hipEvent_t upload_done = nullptr;  // last host-index upload pass
     std::deque<uint64_t> queue;   // QUEUED, FIFO
     std::vector<uint64_t> active;  // ACTIVE, admission order
     std::unordered_map<uint64_t, request> reqs;
@@ -135,8 +138,7 @@ static void free_serve(trove_serve* s) {
         if (s->h_jobs[k]) (void)hipHostFree(s->h_jobs[k]);
         if (s->d_jobs[k]) (void)hipFree(s->d_jobs[k]);
     }
-    for (auto& kv : s->reqs)
-        if (kv.second.upload) (void)hipEventDestroy(kv.second.upload);
+    if (s->upload_done) (void)hipEventDestroy(s->upload_done);
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->out_arena) (void)hipFree(s->out_arena);
     if (s->idx_arena) (void)hipFree(s->idx_arena);
@@ -168,23 +170,34 @@ static int retire(trove_serve* s) {
 }
 
 // Admits queued requests in FIFO order while a slot and an arena range fit.
+// Device-resident indices are read in place; host indices are uploaded into
+// the index arena on the copy stream, and one event after the pass orders
+// them before the next step's kernel.
 static int admit(trove_serve* s) {
+    bool uploaded = false;
     while (!s->queue.empty() && (int)s->active.size() < s->max_slots) {
         request& r = s->reqs[s->queue.front()];
         const int64_t row0 = s->arena->alloc(r.rows);
         if (row0 < 0) break;
         r.row0 = row0;
-        if (r.n > 0) {
+        if (r.idx_on_device) {
+            r.src_idx = r.idx;
+        } else {
             int32_t* dst = s->idx_arena + row0;
-            const hipMemcpyKind kind = r.idx_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-            int rc = hip_status(hipMemcpyAsync(dst, r.idx, sizeof(int32_t) * r.n, kind, s->copy_stream));
-            if (rc == TROVE_OK && !r.upload) rc = hip_status(hipEventCreateWithFlags(&r.upload, hipEventDisableTiming));
-            if (rc == TROVE_OK) rc = hip_status(hipEventRecord(r.upload, s->copy_stream));
+            const int rc = hip_status(
+                hipMemcpyAsync(dst, r.idx, sizeof(int32_t) * r.n, hipMemcpyHostToDevice, s->copy_stream));
             if (rc != TROVE_OK) return rc;
+            r.src_idx = dst;
+            uploaded = true;
         }
         r.state = ACTIVE;
         s->active.push_back(r.id);
         s->queue.pop_front();
+    }
+    if (uploaded) {
+        int rc = hip_status(hipEventRecord(s->upload_done, s->copy_stream));
+        if (rc == TROVE_OK) rc = hip_status(hipStreamWaitEvent(s->stream, s->upload_done, 0));
+        if (rc != TROVE_OK) return rc;
     }
     return TROVE_OK;
 }
@@ -207,18 +220,14 @@ static int launch_step(trove_serve* s) {
             continue;
         }
         const int64_t cnt = left < s->slice ? left : s->slice;
-        if (r.upload && r.scheduled == 0) {
-            const int rc = hip_status(hipStreamWaitEvent(s->stream, r.upload, 0));
-            if (rc != TROVE_OK) return rc;
-        }
         job_desc& d = jobs[nj++];
         d.src = s->table;
-        d.idx = s->idx_arena + r.row0 + r.scheduled;
+        d.idx = r.src_idx + r.scheduled;
         d.dst = s->out_arena + (r.row0 + r.scheduled) * (int64_t)s->row_bytes;
         d.n = cnt;
         d.bytes = s->row_bytes;
         d.variant = s->variant;
-        d.chunk_groups = 8;
+        d.chunk_groups = s->variant == TROVE_VARIANT_SHFL ? 1 : 8;
         d.pad = 0;
         d.first_item = items;
         items += ((cnt + 63) / 64 + d.chunk_groups - 1) / d.chunk_groups;
@@ -283,6 +292,7 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
     s->arena = new (std::nothrow) range_allocator(s->capacity_rows);
     int rc = s->arena ? TROVE_OK : TROVE_ERR_NOMEM;
     if (rc == TROVE_OK) rc = hip_status(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    if (rc == TROVE_OK) rc = hip_status(hipEventCreateWithFlags(&s->upload_done, hipEventDisableTiming));
     if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->out_arena, (size_t)(s->capacity_rows * row_bytes)));
     if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->idx_arena, (size_t)(s->capacity_rows * sizeof(int32_t))));
     for (int k = 0; k < kInflight && rc == TROVE_OK; ++k) {
@@ -367,7 +377,6 @@ int trove_serve_release(trove_serve* s, uint64_t request_id) {
     if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
     if (it->second.state != DONE) return TROVE_ERR_AGAIN;
     if (it->second.row0 >= 0) s->arena->release(it->second.row0, it->second.rows);
-    if (it->second.upload) (void)hipEventDestroy(it->second.upload);
     s->reqs.erase(it);
     return TROVE_OK;
 }
