@@ -395,8 +395,20 @@ def main():
                 nclock.run(f"copy_direct_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="direct"))
                 nclock.run(f"gather_direct_k{k}", (2 * s + 4) * n,
                            lambda: T.gather(src, idx, dst, n, s, variant="direct"))
+            for _ in range(2):
+                nclock.run(f"scatter_tile_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="tile"))
+                nclock.run(f"scatter_shfl_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="shfl"))
+                nclock.run(f"scatter_direct_k{k}", (2 * s + 4) * n,
+                           lambda: T.scatter(src, idx, dst, n, s, variant="direct"))
         for _ in range(3):
             nclock.run("memcpy_ceiling", 2 * n * smax, lambda: T.memcpy_ceiling(src, dst, n * smax))
+        # Block interface: 2^28 32-bit elements, s per lane, converged and
+        # with one wavefront group in eight diverged.
+        e = 1 << 28
+        for sb in (1, 3, 5, 7, 9, 11, 13, 15):
+            for _ in range(2):
+                nclock.run(f"block_s{sb}", 8 * e, lambda: T.block_copy(src, dst, e, sb))
+                nclock.run(f"block_div_s{sb}", 8 * e, lambda: T.block_copy(src, dst, e, sb, divergent=True, seed=7))
         torch.cuda.synchronize()
         nclock.collect()
         for name, (launches, ms, nbytes) in nclock.totals.items():
@@ -415,16 +427,26 @@ def main():
                         "copy_vs_naive": round(row["copy_gbs"] / cd, 2),
                         "gather_tile_vs_naive": round(row["gather_tile_gbs"] / gd, 2),
                         "gather_coalesced_ptr_vs_naive": round(row["gather_coalesced_ptr_gbs"] / gd, 2)})
+            row.update({"scatter_tile_gbs": naive[f"scatter_tile_k{k}"]["gbs"],
+                        "scatter_coalesced_ptr_gbs": naive[f"scatter_shfl_k{k}"]["gbs"],
+                        "scatter_naive_gbs": naive[f"scatter_direct_k{k}"]["gbs"]})
         row["copy_frac_peak"] = round(row["copy_gbs"] / HBM_PEAK_GBS, 3)
         sweep.append(row)
+    block = None
+    if naive:
+        block = {"workload": "block interface copy, 2^28 x 32-bit elements, s per lane; divergent: 1 in 8 "
+                             "64-array groups with lanes masked (warp_converged fallback)",
+                 "gbs": {f"s{sb}": naive[f"block_s{sb}"]["gbs"] for sb in (1, 3, 5, 7, 9, 11, 13, 15)},
+                 "divergent_gbs": {f"s{sb}": naive[f"block_div_s{sb}"]["gbs"] for sb in (1, 3, 5, 7, 9, 11, 13, 15)}}
 
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (profiles/sweep_hbm_traffic.md).
     traffic = None
     traffic_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     if os.path.exists(traffic_path):
         with open(traffic_path) as f:
-            tr = json.load(f)
-        if dname in tr:
-            traffic = tr[dname]
+            tr = json.load(f).get("kernels", {})
+        traffic = tr.get(dname)
 
     del src, dst, idx
     torch.cuda.empty_cache()
@@ -472,9 +494,11 @@ def main():
             "unit": "GB/s",
             "frac": round(d_gbs / HBM_PEAK_GBS, 3),
             "traffic": traffic,
+            "hbm_gbs": round(traffic / (dms / dl * 1e-3) / 1e9, 1) if traffic else None,
         },
         "cpu_baseline": cpu,
         "memcpy_ceiling_gbs": naive.get("memcpy_ceiling", {}).get("gbs"),
+        "block": block,
         "sweep": sweep,
         **extra,
     }

@@ -12,25 +12,41 @@ __device__ __forceinline__ bool divergent_lane(uint64_t seed, int64_t index) {
     return (splitmix64(h + (uint64_t)(index & 63)) & 1) != 0;
 }
 
+// UNROLL 64-array groups per 64-thread work-group (one-shot grid; 4 / S
+// groups for arrays under 12 bytes so a wavefront moves about 1 KiB),
+// XCD-contiguous unit order from 20-byte arrays up, streaming accesses. The
+// partial last group has inactive lanes, so warp_converged() is false there
+// and it takes the fallback, as in the reference's test.
 template <int S>
-__global__ __launch_bounds__(kBlock) void k_block_copy(const uint32_t* __restrict__ x, uint32_t* __restrict__ r,
-                                                       int64_t l, int64_t e, int divergent, uint64_t seed) {
+struct block_shape {
+    static constexpr int unroll = S == 1 ? 4 : S == 2 ? 2 : 1;
+};
+
+template <int S>
+__global__ __launch_bounds__(64) void k_block_copy(const uint32_t* __restrict__ x, uint32_t* __restrict__ r, int64_t l,
+                                                   int64_t e, int divergent, uint64_t seed) {
     typedef trove::array<uint32_t, S> s_ary;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    const int64_t gi = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    for (int64_t index = gi; index < l; index += stride) {
-        if (divergent && divergent_lane(seed, index)) {
-            const s_ary d = trove::load_array<S>(x, index);
-            trove::store_array(r, index, d);
-        } else if (trove::warp_converged()) {
-            const s_ary d = trove::load_array_warp_contiguous<S>(x, index);
-            trove::store_array_warp_contiguous(r, index, d);
-        } else {
-            const s_ary d = trove::load_array<S>(x, index);
-            trove::store_array(r, index, d);
+    constexpr int U = block_shape<S>::unroll;
+    const int64_t unit = S >= 5 ? unit_for_xcd(blockIdx.x, (int64_t)gridDim.x) : (int64_t)blockIdx.x;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t index = ((unit * U + u) << 6) + threadIdx.x;
+        if (index < l) {
+            if (divergent && divergent_lane(seed, index)) {
+                const s_ary d = trove::load_array<S>(x, index);
+                trove::store_array(r, index, d);
+            } else if (trove::warp_converged()) {
+                const s_ary d = trove::load_array_warp_contiguous<S, trove::WARP_SIZE, trove::streaming>(x, index);
+                trove::store_array_warp_contiguous<S, trove::WARP_SIZE, trove::streaming>(r, index, d);
+            } else {
+                const s_ary d = trove::load_array<S>(x, index);
+                trove::store_array(r, index, d);
+            }
         }
     }
-    for (int64_t j = l * S + gi; j < e; j += stride) r[j] = x[j];
+    // Trailing e - l*S elements, one per lane of the first work-group.
+    const int64_t j = l * S + (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (j < e) r[j] = x[j];
 }
 
 template <int S>
@@ -51,8 +67,12 @@ template <int S>
 static int launch_block_copy(const uint32_t* x, uint32_t* r, int64_t e, int divergent, uint64_t seed,
                              hipStream_t s) {
     const int64_t l = e / S;
-    const int64_t blocks = grid_for((l + 63) / 64 > 0 ? (l + 63) / 64 : 1);
-    hipLaunchKernelGGL((k_block_copy<S>), dim3((unsigned)blocks), dim3(kBlock), 0, s, x, r, l, e, divergent, seed);
+    // One work-group per 64-array group; at least enough lanes for the
+    // e - l*S < S <= 64 trailing elements.
+    constexpr int U = block_shape<S>::unroll;
+    const int64_t groups = (l + 63) / 64;
+    const int64_t blocks = grid_for(groups > 0 ? (groups + U - 1) / U : 1, 1);
+    hipLaunchKernelGGL((k_block_copy<S>), dim3((unsigned)blocks), dim3(64), 0, s, x, r, l, e, divergent, seed);
     return launch_status();
 }
 

@@ -66,6 +66,18 @@ __device__ __forceinline__ int64_t wave_count() {
     return (int64_t)gridDim.x * (B / 64);
 }
 
+// XCD-contiguous work mapping. MI355X dispatches consecutive work-groups to
+// its 8 XCDs round-robin, so in dispatch order each XCD's L2 sees every 8th
+// unit. unit_for_xcd maps work-group b of a one-shot grid of `total` units to
+// a unit such that XCD x (= b % 8) walks a contiguous range of about
+// total / 8 units. It is a bijection on [0, total) for any total.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t unit_for_xcd(int64_t b, int64_t total) {
+    const int64_t x = b % kXcds, j = b / kXcds;
+    const int64_t base = total / kXcds, extra = total % kXcds;
+    return x * base + (x < extra ? x : extra) + j;
+}
+
 __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;

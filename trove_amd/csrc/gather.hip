@@ -2,7 +2,8 @@
 //  SHFL   - the reference benchmark's coalesced_ptr kernels
 //           (tests/benchmark.cu:90-110): per-lane index read, struct read and
 //           struct write through coalesced_ptr (tile cascade, address
-//           broadcast, R2C / C2R, contiguity fast path on the contiguous side).
+//           broadcast, R2C / C2R, contiguity fast path on the contiguous side);
+//           the scatter assigns one coalesced_ref to another (ptr.h:47-72).
 //  TILE   - the same tile-indexed word accesses with the transposition
 //           elided: the gathered words already sit in the column-side layout
 //           a coalesced tile store takes (and scatter mirrors it).
@@ -47,8 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_shfl(const T* __restrict__ s
     if (i < n) {
         trove::coalesced_ptr<T> s(const_cast<T*>(src));
         trove::coalesced_ptr<T> r(dst);
-        const T v = s[i];
-        r[idx[i]] = v;
+        r[idx[i]] = s[i];  // coalesced_ref = coalesced_ref
     }
 }
 
