@@ -22,10 +22,33 @@ _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64 = ctypes.c_int64
 
 
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libref_parts.so")
+REFERENCE_DIR = "/root/reference"
+
+
 def build() -> str:
-    """Compiles oracle.c with the committed Makefile; returns the library path."""
+    """Compiles oracle.c with the committed Makefile and, when the reference
+    sources are present, the reference's pure headers into oracle/_ref/;
+    returns the oracle library path."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if os.path.isdir(os.path.join(REFERENCE_DIR, "trove")):
+        subprocess.run(["make", "-s", "-C", HERE, "ref", f"REFERENCE={REFERENCE_DIR}"], check=True)
     return LIB_PATH
+
+
+def ref_lib():
+    """The reference-built parts (oracle/_ref), or None when not built."""
+    if not os.path.exists(REF_LIB_PATH):
+        return None
+    L = ctypes.CDLL(REF_LIB_PATH)
+    for name in ("ref_gcd_table", "ref_mod_inverse_table", "ref_log_table"):
+        getattr(L, name).argtypes = [_i32p]
+        getattr(L, name).restype = None
+    L.ref_rotate.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, _i32p]
+    L.ref_rotate.restype = ctypes.c_int
+    L.ref_counting_sum.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p]
+    L.ref_counting_sum.restype = ctypes.c_int
+    return L
 
 
 def lib():

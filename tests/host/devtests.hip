@@ -1,0 +1,179 @@
+// Device-side unit checks of the public device API, built into
+// tests/host/libtrove_devtests.so and driven by tests/test_gpu_device_api.py.
+// Each entry point runs one 64-lane work-group and writes what the lanes saw;
+// the host test computes the expected values.
+#include <hip/hip_runtime.h>
+#include <trove/aos.h>
+#include <trove/ptr.h>
+
+#include <cstdint>
+
+namespace {
+
+struct w16 {
+    uint32_t a, b, c, d;
+};
+
+template <typename T>
+__device__ T encode(uint32_t v) {
+    T t;
+    uint32_t w[sizeof(T) / 4];
+    for (int x = 0; x < (int)(sizeof(T) / 4); ++x) w[x] = v * 4u + (uint32_t)x;
+    __builtin_memcpy(&t, w, sizeof(T));
+    return t;
+}
+
+template <typename T>
+__device__ uint32_t decode(const T& t) {
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &t, sizeof(T));
+    // Every word must agree with the first; a mismatch is reported as ~0.
+    for (int x = 1; x < (int)(sizeof(T) / 4); ++x)
+        if (w[x] != w[0] + (uint32_t)x) return ~0u;
+    return w[0] / 4u;
+}
+
+// Lane id of tile rank r: column-side value r + j*N goes in, the public
+// explicit-index R2C must produce r*M + j; then the self-contained C2R must
+// restore the input.
+template <int N, int M, typename T>
+__global__ void k_transpose(uint32_t* r2c_out, uint32_t* c2r_out) {
+    trove::thread_tile<N> tile;
+    const int r = tile.id();
+    const int lane = threadIdx.x;
+    T in[M];
+    for (int j = 0; j < M; ++j) in[j] = encode<T>((uint32_t)(r + j * N));
+    trove::array<T, M> v = trove::make_array<T, M>(in);
+    trove::array<int, M> idx;
+    int rot;
+    trove::r2c_compute_indices(idx, rot, tile);
+    trove::r2c_warp_transpose(v, idx, rot, tile);
+    T out[M];
+    trove::make_carray<T, M>(v, out);
+    for (int j = 0; j < M; ++j) r2c_out[lane * M + j] = decode(out[j]);
+    trove::c2r_compute_indices(idx, rot, tile);
+    trove::c2r_warp_transpose(v, idx, rot, tile);
+    trove::make_carray<T, M>(v, out);
+    for (int j = 0; j < M; ++j) c2r_out[lane * M + j] = decode(out[j]);
+}
+
+template <int M>
+__global__ void k_rotate(uint32_t* out) {
+    const int lane = threadIdx.x;
+    int lim = 1;
+    while (lim < M) lim <<= 1;
+    const int a = lane % lim;
+    uint32_t t[M];
+    for (int j = 0; j < M; ++j) t[j] = 100u * (uint32_t)lane + (uint32_t)j;
+    trove::array<uint32_t, M> v = trove::make_array<uint32_t, M>(t);
+    v = trove::rotate(v, a);
+    trove::make_carray<uint32_t, M>(v, t);
+    for (int j = 0; j < M; ++j) out[lane * M + j] = t[j];
+}
+
+__global__ void k_converged(uint64_t mask, int32_t* out) {
+    const int lane = threadIdx.x;
+    for (int k = 0; k < 6; ++k) out[lane * 6 + k] = -1;
+    if ((mask >> lane) & 1ull) {
+        out[lane * 6 + 0] = trove::warp_converged<64>();
+        out[lane * 6 + 1] = trove::warp_converged<32>();
+        out[lane * 6 + 2] = trove::warp_converged<16>();
+        out[lane * 6 + 3] = trove::warp_converged<8>();
+        out[lane * 6 + 4] = trove::warp_converged<4>();
+        out[lane * 6 + 5] = trove::warp_converged<2>();
+    }
+}
+
+template <int N>
+__global__ void k_shfl(const int* src_rank, uint32_t* out) {
+    trove::thread_tile<N> tile;
+    const int lane = threadIdx.x;
+    const uint64_t v = (uint64_t)lane * 0x100000001ull + 7;
+    const uint64_t got = tile.shfl(v, src_rank[lane]);
+    out[2 * lane] = (uint32_t)got;
+    out[2 * lane + 1] = (uint32_t)(got >> 32);
+}
+
+// Active lanes (mask) copy src[perm[lane]] to dst[lane] (gather) or
+// src[lane] to dst[perm[lane]] (scatter) through trove::load / trove::store.
+template <int S>
+__global__ void k_cascade(uint64_t mask, int scatter, const trove::array<uint32_t, S / 4>* src, const int* perm,
+                          trove::array<uint32_t, S / 4>* dst) {
+    const int lane = threadIdx.x;
+    if ((mask >> lane) & 1ull) {
+        if (scatter) {
+            trove::store(trove::load(src + lane), dst + perm[lane]);
+        } else {
+            trove::store(trove::load(src + perm[lane]), dst + lane);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Returns 0 and fills r2c_out / c2r_out (64 * m words each), -1 for an
+// unsupported combination, -2 on a launch error.
+int devtest_transpose(int n, int m, int word_bytes, uint32_t* r2c_out, uint32_t* c2r_out) {
+#define TT(N, M, T)                                                                \
+    if (n == N && m == M && word_bytes == (int)sizeof(T)) {                        \
+        hipLaunchKernelGGL((k_transpose<N, M, T>), dim3(1), dim3(64), 0, 0, r2c_out, c2r_out); \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                      \
+    }
+#define TT_M(N, T) TT(N, 2, T) TT(N, 3, T) TT(N, 4, T) TT(N, 5, T) TT(N, 6, T) TT(N, 7, T) TT(N, 8, T) \
+    TT(N, 9, T) TT(N, 10, T) TT(N, 12, T) TT(N, 14, T) TT(N, 15, T) TT(N, 16, T)
+    TT_M(64, uint32_t)
+    TT_M(32, uint32_t)
+    TT_M(16, uint32_t)
+    TT(8, 2, uint32_t) TT(8, 3, uint32_t) TT(8, 5, uint32_t) TT(8, 6, uint32_t) TT(8, 8, uint32_t)
+    TT(4, 2, uint32_t) TT(4, 3, uint32_t) TT(4, 4, uint32_t)
+    TT(2, 2, uint32_t)
+    TT(64, 2, uint64_t) TT(64, 3, uint64_t) TT(64, 6, uint64_t) TT(64, 12, uint64_t)
+    TT(64, 2, w16) TT(64, 5, w16) TT(64, 6, w16) TT(64, 16, w16) TT(32, 6, w16)
+    TT(64, 24, uint32_t) TT(64, 33, uint32_t) TT(64, 64, uint32_t)
+#undef TT_M
+#undef TT
+    return -1;
+}
+
+int devtest_rotate(int m, uint32_t* out) {
+#define RT(M)                                                                    \
+    if (m == M) {                                                                \
+        hipLaunchKernelGGL((k_rotate<M>), dim3(1), dim3(64), 0, 0, out);         \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                    \
+    }
+    RT(1) RT(2) RT(3) RT(4) RT(5) RT(6) RT(7) RT(8) RT(9) RT(11) RT(13) RT(15) RT(16)
+#undef RT
+    return -1;
+}
+
+int devtest_converged(uint64_t mask, int32_t* out) {
+    hipLaunchKernelGGL(k_converged, dim3(1), dim3(64), 0, 0, mask, out);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -2;
+}
+
+int devtest_shfl(int n, const int* src_rank, uint32_t* out) {
+#define SH(N)                                                                    \
+    if (n == N) {                                                                \
+        hipLaunchKernelGGL((k_shfl<N>), dim3(1), dim3(64), 0, 0, src_rank, out); \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                    \
+    }
+    SH(64) SH(32) SH(16) SH(8) SH(4) SH(2)
+#undef SH
+    return -1;
+}
+
+int devtest_cascade(int s, uint64_t mask, int scatter, const void* src, const int* perm, void* dst) {
+#define CS(S)                                                                                        \
+    if (s == S) {                                                                                    \
+        hipLaunchKernelGGL((k_cascade<S>), dim3(1), dim3(64), 0, 0, mask, scatter,                  \
+                           (const trove::array<uint32_t, S / 4>*)src, perm, (trove::array<uint32_t, S / 4>*)dst); \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                                        \
+    }
+    CS(8) CS(12) CS(20) CS(32) CS(48) CS(60) CS(96)
+#undef CS
+    return -1;
+}
+
+}  // extern "C"

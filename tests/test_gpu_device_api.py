@@ -1,0 +1,137 @@
+"""GPU unit checks of the public device API (tests/host/devtests.hip).
+
+* r2c_compute_indices + r2c_warp_transpose (explicit indices) map the
+  column-side layout to the row-side one and c2r_* restores it, for tile
+  widths 64..2, odd / power-of-two / composite m and 4-, 8- and 16-byte
+  elements (closed-form post-condition, trove/transpose.h:629-696);
+* rotate with a per-lane amount equals np.roll (rotate.h:111-115);
+* warp_converged<N> under arbitrary EXEC masks equals the truth table of
+  "every lane of my aligned N-group is active" (warp.h:56-93);
+* thread_tile<N>::shfl reads from the addressed rank of the caller's tile;
+* trove::load / trove::store under arbitrary EXEC masks move exactly the
+  active lanes' structs (the tile cascade, aos.h:258-290).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "host", "libtrove_devtests.so")
+
+
+@pytest.fixture(scope="module")
+def dt(cuda):
+    assert os.path.exists(LIB), f"{LIB} not built (run __graft_entry__.build())"
+    L = ctypes.CDLL(LIB)
+    vp, i, u64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
+    L.devtest_transpose.argtypes = [i, i, i, vp, vp]
+    L.devtest_rotate.argtypes = [i, vp]
+    L.devtest_converged.argtypes = [u64, vp]
+    L.devtest_shfl.argtypes = [i, vp, vp]
+    L.devtest_cascade.argtypes = [i, u64, i, vp, vp, vp]
+    for f in (L.devtest_transpose, L.devtest_rotate, L.devtest_converged, L.devtest_shfl, L.devtest_cascade):
+        f.restype = i
+    return L
+
+
+TRANSPOSE_CASES = (
+    [(n, m, 4) for n in (64, 32, 16) for m in (2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16) if m <= n]
+    + [(8, m, 4) for m in (2, 3, 5, 6, 8)] + [(4, 2, 4), (4, 3, 4), (4, 4, 4), (2, 2, 4)]
+    + [(64, m, 8) for m in (2, 3, 6, 12)] + [(64, m, 16) for m in (2, 5, 6, 16)] + [(32, 6, 16)]
+    + [(64, 24, 4), (64, 33, 4), (64, 64, 4)]
+)
+
+
+@pytest.mark.parametrize("n,m,wb", TRANSPOSE_CASES)
+def test_public_transpose_api(dt, n, m, wb):
+    r2c = torch.zeros(64 * m, dtype=torch.int32, device="cuda")
+    c2r = torch.zeros(64 * m, dtype=torch.int32, device="cuda")
+    assert dt.devtest_transpose(n, m, wb, r2c.data_ptr(), c2r.data_ptr()) == 0
+    rank = np.arange(64) % n
+    want_row = (rank[:, None] * m + np.arange(m)[None, :]).astype(np.int64)
+    want_col = (rank[:, None] + np.arange(m)[None, :] * n).astype(np.int64)
+    got_r2c = r2c.cpu().numpy().view(np.uint32).reshape(64, m).astype(np.int64)
+    got_c2r = c2r.cpu().numpy().view(np.uint32).reshape(64, m).astype(np.int64)
+    assert np.array_equal(got_r2c, want_row), (n, m, wb)
+    assert np.array_equal(got_c2r, want_col), (n, m, wb)
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 13, 15, 16])
+def test_rotate_per_lane(dt, m):
+    out = torch.zeros(64 * m, dtype=torch.int32, device="cuda")
+    assert dt.devtest_rotate(m, out.data_ptr()) == 0
+    got = out.cpu().numpy().reshape(64, m)
+    lim = 1
+    while lim < m:
+        lim <<= 1
+    for lane in range(64):
+        t = 100 * lane + np.arange(m)
+        assert np.array_equal(got[lane], np.roll(t, -((lane % lim) % m))), (m, lane)
+
+
+def _masks():
+    rng = np.random.default_rng(5)
+    fixed = [0xFFFFFFFFFFFFFFFF, 0x00000000FFFFFFFF, 0xFFFFFFFF00000000, 0x0F0F0F0F0F0F0F0F,
+             0x5555555555555555, 0x1, 0x3, 0xFFFF0000FFFF0000, 0x00FF00FF00FF00FF, 0x8000000000000001]
+    return fixed + [int(x) for x in rng.integers(0, 2**63, size=20, dtype=np.int64)] + \
+        [(1 << 64) - 1 - (1 << int(b)) for b in rng.integers(0, 64, size=5)]
+
+
+@pytest.mark.parametrize("mask", _masks())
+def test_warp_converged_truth_table(dt, mask):
+    out = torch.zeros(64 * 6, dtype=torch.int32, device="cuda")
+    assert dt.devtest_converged(mask, out.data_ptr()) == 0
+    got = out.cpu().numpy().reshape(64, 6)
+    active = [(mask >> l) & 1 for l in range(64)]
+    for lane in range(64):
+        if not active[lane]:
+            assert (got[lane] == -1).all()
+            continue
+        for k, N in enumerate((64, 32, 16, 8, 4, 2)):
+            base = lane & ~(N - 1)
+            want = int(all(active[base:base + N]))
+            assert got[lane, k] == want, (hex(mask), lane, N)
+
+
+@pytest.mark.parametrize("n", [64, 32, 16, 8, 4, 2])
+def test_tile_shfl(dt, n):
+    src = np.random.default_rng(n).integers(0, n, size=64).astype(np.int32)
+    d_src = torch.from_numpy(src).cuda()
+    out = torch.zeros(128, dtype=torch.int32, device="cuda")
+    assert dt.devtest_shfl(n, d_src.data_ptr(), out.data_ptr()) == 0
+    got = out.cpu().numpy().view(np.uint32).reshape(64, 2)
+    for lane in range(64):
+        from_lane = (lane & ~(n - 1)) + int(src[lane])
+        v = from_lane * 0x100000001 + 7
+        assert got[lane, 0] == v & 0xFFFFFFFF and got[lane, 1] == v >> 32, (n, lane)
+
+
+@pytest.mark.parametrize("s", [8, 12, 20, 32, 48, 60, 96])
+def test_cascade_under_masks(dt, s):
+    rng = np.random.default_rng(s)
+    k = s // 4
+    for mask in _masks()[:18]:
+        for scatter in (0, 1):
+            for perm in (np.arange(64), rng.permutation(64)):
+                perm = perm.astype(np.int32)
+                src = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
+                init = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
+                d_src = torch.from_numpy(src.view(np.int32)).cuda()
+                d_dst = torch.from_numpy(init.view(np.int32)).cuda()
+                d_perm = torch.from_numpy(perm).cuda()
+                assert dt.devtest_cascade(s, mask, scatter, d_src.data_ptr(), d_perm.data_ptr(),
+                                          d_dst.data_ptr()) == 0
+                got = d_dst.cpu().numpy().view(np.uint32).reshape(64, k)
+                want = init.reshape(64, k).copy()
+                src2 = src.reshape(64, k)
+                for lane in range(64):
+                    if (mask >> lane) & 1:
+                        if scatter:
+                            want[perm[lane]] = src2[lane]
+                        else:
+                            want[lane] = src2[perm[lane]]
+                assert np.array_equal(got, want), (s, hex(mask), scatter)

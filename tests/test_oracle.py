@@ -144,3 +144,59 @@ def test_schedule_check_detects_a_planted_mistake():
     wrong[[3, 4]] = wrong[[4, 3]]
     row = np.array([[l * m + j for j in range(m)] for l in range(n)], dtype=np.int32)
     assert not np.array_equal(oracle.r2c_tile(wrong), row)
+
+
+# ---- against the reference's own code (oracle/_ref, built from the
+# reference headers in place where they compile: number theory, rotate,
+# counting_array / sum) ----
+
+def _ref():
+    L = oracle.ref_lib()
+    if L is None:
+        pytest.skip("oracle/_ref not built (the reference sources are not present here)")
+    return L
+
+
+def _tab(fn, shape):
+    out = np.zeros(shape, dtype=np.int32)
+    fn(out.ctypes.data_as(oracle._i32p))
+    return out
+
+
+def test_gcd_and_inverse_match_reference_build():
+    R, L = _ref(), oracle.lib()
+    g = _tab(R.ref_gcd_table, 33 * 33).reshape(33, 33)
+    inv = _tab(R.ref_mod_inverse_table, 33 * 33).reshape(33, 33)
+    lg = _tab(R.ref_log_table, 33)
+    for u in range(1, 33):
+        for v in range(1, 33):
+            assert L.oracle_gcd(u, v) == g[u, v], (u, v)
+    for a in range(0, 33):
+        for m in range(1, 33):
+            assert L.oracle_mod_inverse(a, m) == inv[a, m], (a, m)
+    for m in range(1, 33):
+        assert L.oracle_floor_log2(m) == lg[m]
+
+
+@pytest.mark.parametrize("m", range(1, 17))
+def test_rotate_matches_reference_build(m):
+    R = _ref()
+    t = np.arange(7, 7 + m, dtype=np.int32)
+    limit = 1 << max(1, (m - 1).bit_length())
+    for a in range(limit):
+        r = np.zeros(m, dtype=np.int32)
+        assert R.ref_rotate(m, t.ctypes.data_as(oracle._i32p), a, r.ctypes.data_as(oracle._i32p)) == 0
+        assert np.array_equal(oracle.rotate(t, a), r), (m, a)
+
+
+def test_counting_and_sum_match_reference_build():
+    R = _ref()
+    n = 64 * 2 + 17
+    for k in range(1, 17):
+        words = np.zeros(k, dtype=np.int32)
+        mine = oracle.counting_store(n, k)
+        sums = oracle.load_sum(mine, n, k)
+        for g in (0, 1, 63, 64, n - 1):
+            s = R.ref_counting_sum(k, g * k, 1, words.ctypes.data_as(oracle._i32p))
+            assert np.array_equal(words.view(np.uint32), mine[g * k:(g + 1) * k]), (k, g)
+            assert np.uint32(s & 0xFFFFFFFF) == sums[g], (k, g)

@@ -99,6 +99,22 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB_PATH
 
 
+DEVTESTS_SRC = os.path.join(ROOT, "tests", "host", "devtests.hip")
+DEVTESTS_LIB = os.path.join(ROOT, "tests", "host", "libtrove_devtests.so")
+
+
+def build_devtests() -> str:
+    """Compiles the device-API unit-test kernels (tests/host/devtests.hip)."""
+    tmp = DEVTESTS_LIB + ".tmp"
+    rc, cmd, out = _compile([HIPCC, *HIP_FLAGS, "-shared", DEVTESTS_SRC, "-o", tmp])
+    if rc != 0:
+        print(cmd)
+        print(out)
+        raise RuntimeError("device test library failed to compile")
+    os.replace(tmp, DEVTESTS_LIB)
+    return DEVTESTS_LIB
+
+
 if __name__ == "__main__":
     path = build(verbose="-v" in sys.argv)
     print(path)
