@@ -72,7 +72,8 @@ static int launch_block_copy(const uint32_t* x, uint32_t* r, int64_t e, int dive
     constexpr int U = block_shape<S>::unroll;
     const int64_t groups = (l + 63) / 64;
     const int64_t blocks = grid_for(groups > 0 ? (groups + U - 1) / U : 1, 1);
-    hipLaunchKernelGGL((k_block_copy<S>), dim3((unsigned)blocks), dim3(64), 0, s, x, r, l, e, divergent, seed);
+    const size_t lds = stream_lds_reservation((int64_t)U * 64 * 4 * S);
+    hipLaunchKernelGGL((k_block_copy<S>), dim3((unsigned)blocks), dim3(64), lds, s, x, r, l, e, divergent, seed);
     return launch_status();
 }
 

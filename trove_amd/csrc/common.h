@@ -95,6 +95,16 @@ int device_cu_count();                // compute units of the current device
 // stride for the (unreachable in practice) case of more than 2^31 groups.
 int64_t grid_for(int64_t waves, int waves_per_block = kWavesPerBlock);
 
+// Dynamic LDS to reserve per work-group so that at most about
+// kStreamWindowBytes / bytes_per_block bytes of streaming work sit on one CU
+// at a time (work-groups per CU = LDS per CU / reservation). Measured on
+// MI355X: a resident window of ~32-40 KiB per CU streams best (4-KiB-per-
+// wavefront copies: 10 wavefronts per CU reach 6.5-6.7 TB/s against
+// 5.9-6.1 TB/s at full occupancy; 1-KiB wavefronts want no cap). Returns 0
+// when no cap is needed.
+constexpr int64_t kStreamWindowBytes = 40 * 1024;
+size_t stream_lds_reservation(int64_t bytes_per_block);
+
 inline bool aligned(const void* p, int bytes) { return ((uintptr_t)p % (uintptr_t)bytes) == 0; }
 
 template <int B>

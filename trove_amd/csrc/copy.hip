@@ -18,6 +18,7 @@ namespace trove_amd {
 
 // XCD-contiguous mapping (unit_for_xcd) for structs of 20 bytes and more:
 // +7-9% at 36-60 bytes, but -13% at 12 bytes, where dispatch order wins.
+// The launch also caps resident work-groups per CU (stream_lds_reservation).
 template <typename T>
 struct copy_shape {
     static constexpr int words = sizeof(T) / 4 > 0 ? (int)(sizeof(T) / 4) : 1;
@@ -85,8 +86,9 @@ static int launch_copy_shfl(const T* src, T* dst, int64_t n, hipStream_t s) {
     typedef copy_shape<T> C;
     const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
     const int64_t blocks = grid_for(units, C::block / 64);
+    const size_t lds = stream_lds_reservation((int64_t)C::unroll * 64 * (int64_t)sizeof(T) * (C::block / 64));
     hipLaunchKernelGGL((k_copy_shfl<T, TILE, C::unroll, C::block, C::xcd_map>), dim3((unsigned)blocks),
-                       dim3(C::block), 0, s, src, dst, n);
+                       dim3(C::block), lds, s, src, dst, n);
     return launch_status();
 }
 
