@@ -64,15 +64,21 @@ def parse_k(spec: str) -> list[int]:
 
 
 class KernelClock:
-    """HIP events around each launch on the launch stream (torch's current)."""
+    """HIP events around each launch on the launch stream (torch's current).
+    The first launch of every name is untimed: HIP loads a kernel's code
+    object lazily at its first launch."""
 
-    def __init__(self, torch):
+    def __init__(self, torch, seen=None):
         self.torch = torch
         self.pending = []  # (name, bytes, start, end)
         self.totals = {}   # name -> [launches, ms, bytes]
+        self.seen = set() if seen is None else seen
 
     def run(self, name, nbytes, fn):
         t = self.torch
+        if name not in self.seen:
+            self.seen.add(name)
+            fn()
         s, e = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
         s.record()
         fn()
@@ -362,7 +368,9 @@ def main():
     torch.cuda.synchronize()
     warm.collect()
 
-    clock = KernelClock(torch)
+    clock = KernelClock(torch, seen=warm.seen)
+    if args.warmup == 0:  # load the code objects outside the timed region
+        step(KernelClock(torch, seen=clock.seen))
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for i in range(args.steps):
@@ -432,6 +440,31 @@ def main():
                         "scatter_naive_gbs": naive[f"scatter_direct_k{k}"]["gbs"]})
         row["copy_frac_peak"] = round(row["copy_gbs"] / HBM_PEAK_GBS, 3)
         sweep.append(row)
+    # Structs beyond the sweep: 96..256 bytes (6..16 16-byte words; 96, 160,
+    # 192 and 224 bytes take the composite schedule), N = 2^24.
+    large = None
+    if naive:
+        nl = 1 << 24
+        # 2^24 rows of up to 256 bytes = the 4 GiB of src / dst; indices are
+        # their own permutation of [0, 2^24).
+        assert nl * 256 <= src.numel() * 4 and nl * 256 <= dst.numel() * 4
+        idx_l = torch.from_numpy(T.gen_permutation(nl, args.seed + 31)).to(dev)
+        lclock = KernelClock(torch)
+        for sb in (96, 128, 160, 192, 224, 256):
+            for _ in range(3):
+                lclock.run(f"copy_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="shfl"))
+                lclock.run(f"copy_naive_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="direct"))
+                lclock.run(f"gather_{sb}", (2 * sb + 4) * nl,
+                           lambda: T.gather(src, idx_l, dst, nl, sb, variant="tile"))
+                lclock.run(f"gather_naive_{sb}", (2 * sb + 4) * nl,
+                           lambda: T.gather(src, idx_l, dst, nl, sb, variant="direct"))
+        torch.cuda.synchronize()
+        lclock.collect()
+        del idx_l
+        large = {"workload": "struct sizes 96..256 B (composite schedule at 96/160/192/224), N=2^24, "
+                             "gather through a seeded permutation of [0, 2^24)"}
+        for name, (launches, ms, nbytes) in lclock.totals.items():
+            large[name + "_gbs"] = round(nbytes / launches / (ms / launches * 1e-3) / 1e9, 1)
     block = None
     if naive:
         block = {"workload": "block interface copy, 2^28 x 32-bit elements, s per lane; divergent: 1 in 8 "
@@ -524,6 +557,7 @@ def main():
         "cpu_baseline": cpu,
         "memcpy_ceiling_gbs": naive.get("memcpy_ceiling", {}).get("gbs"),
         "block": block,
+        "large_structs": large,
         "sweep": sweep,
         **extra,
     }

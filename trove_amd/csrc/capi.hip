@@ -95,6 +95,10 @@ static const size_entry* find_size(int bytes) {
 
 static bool valid_variant(int v) { return v >= TROVE_VARIANT_AUTO && v <= TROVE_VARIANT_TILE; }
 
+// Largest n the one-unit-per-wavefront kernels cover with a grid of at most
+// 2^31 - 1 single-wavefront work-groups (2^37 structs, far beyond any HBM).
+constexpr int64_t kMaxStructs = (int64_t)0x7fffffff * 64;
+
 }  // namespace trove_amd
 
 using namespace trove_amd;
@@ -130,7 +134,7 @@ int trove_struct_word_bytes(int struct_bytes) { return struct_bytes > 0 ? word_b
 int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int variant, int tile, void* stream) {
     const size_entry* e = find_size(struct_bytes);
     if (!e) return TROVE_ERR_SIZE;
-    if (n < 0) return TROVE_ERR_INVALID;
+    if (n < 0 || n > kMaxStructs) return TROVE_ERR_INVALID;
     if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE) return TROVE_ERR_VARIANT;
     if (tile != 0 && (tile < 2 || tile > 64 || (tile & (tile - 1)) != 0)) return TROVE_ERR_VARIANT;
     if (n == 0) return TROVE_OK;
@@ -140,31 +144,37 @@ int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int vari
     return e->copy(src, dst, n, variant, tile, static_cast<hipStream_t>(stream));
 }
 
+// Argument checks shared by gather and scatter. *launch is false when there
+// is nothing to do (n == 0).
 static int index_common(const size_entry* e, const void* src, const int32_t* idx, void* dst, int64_t n,
-                        int struct_bytes, int variant) {
+                        int struct_bytes, int variant, bool* launch) {
+    *launch = false;
     if (!e) return TROVE_ERR_SIZE;
-    if (n < 0) return TROVE_ERR_INVALID;
+    if (n < 0 || n > kMaxStructs) return TROVE_ERR_INVALID;
     if (!valid_variant(variant)) return TROVE_ERR_VARIANT;
-    if (n == 0) return TROVE_OK + 1;  // nothing to launch
+    if (n == 0) return TROVE_OK;
     if (!src || !dst || !idx) return TROVE_ERR_INVALID;
     const int w = word_bytes_of(struct_bytes);
     if (!aligned(src, w) || !aligned(dst, w) || !aligned(idx, 4)) return TROVE_ERR_ALIGN;
+    *launch = true;
     return TROVE_OK;
 }
 
 int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
                  void* stream) {
     const size_entry* e = find_size(struct_bytes);
-    const int rc = index_common(e, src, idx, dst, n, struct_bytes, variant);
-    if (rc != TROVE_OK) return rc > 0 ? TROVE_OK : rc;
+    bool launch = false;
+    const int rc = index_common(e, src, idx, dst, n, struct_bytes, variant, &launch);
+    if (rc != TROVE_OK || !launch) return rc;
     return e->gather(src, idx, dst, n, variant, static_cast<hipStream_t>(stream));
 }
 
 int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
                   void* stream) {
     const size_entry* e = find_size(struct_bytes);
-    const int rc = index_common(e, src, idx, dst, n, struct_bytes, variant);
-    if (rc != TROVE_OK) return rc > 0 ? TROVE_OK : rc;
+    bool launch = false;
+    const int rc = index_common(e, src, idx, dst, n, struct_bytes, variant, &launch);
+    if (rc != TROVE_OK || !launch) return rc;
     return e->scatter(src, idx, dst, n, variant, static_cast<hipStream_t>(stream));
 }
 
@@ -192,7 +202,7 @@ int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, 
 
 int trove_block_copy(const void* src, void* dst, int64_t n_elems, int s, int divergent, uint64_t seed,
                      void* stream) {
-    if (s < 1 || s > 16 || n_elems < 0) return TROVE_ERR_INVALID;
+    if (s < 1 || s > 16 || n_elems < 0 || n_elems / s > kMaxStructs) return TROVE_ERR_INVALID;
     if (n_elems == 0) return TROVE_OK;
     if (!src || !dst) return TROVE_ERR_INVALID;
     const int w = word_bytes_of(4 * s);
@@ -202,7 +212,7 @@ int trove_block_copy(const void* src, void* dst, int64_t n_elems, int s, int div
 }
 
 int trove_block_write(void* dst, int64_t n_arrays, int s, void* stream) {
-    if (s < 1 || s > 16 || n_arrays < 0) return TROVE_ERR_INVALID;
+    if (s < 1 || s > 16 || n_arrays < 0 || n_arrays > kMaxStructs) return TROVE_ERR_INVALID;
     if (n_arrays == 0) return TROVE_OK;
     if (!dst) return TROVE_ERR_INVALID;
     if (!aligned(dst, word_bytes_of(4 * s))) return TROVE_ERR_ALIGN;
