@@ -440,29 +440,30 @@ def main():
                         "scatter_naive_gbs": naive[f"scatter_direct_k{k}"]["gbs"]})
         row["copy_frac_peak"] = round(row["copy_gbs"] / HBM_PEAK_GBS, 3)
         sweep.append(row)
-    # Structs beyond the sweep: 96..256 bytes (6..16 16-byte words; 96, 160,
-    # 192 and 224 bytes take the composite schedule), N = 2^24.
+    # Structs beyond the sweep: 96..1024 bytes (6..64 16-byte words; 96, 160,
+    # 192, 224, 384 and 768 bytes take the composite schedule), N = 2^24 up
+    # to 256 bytes and 2^22 above.
     large = None
     if naive:
-        nl = 1 << 24
-        # 2^24 rows of up to 256 bytes = the 4 GiB of src / dst; indices are
-        # their own permutation of [0, 2^24).
-        assert nl * 256 <= src.numel() * 4 and nl * 256 <= dst.numel() * 4
-        idx_l = torch.from_numpy(T.gen_permutation(nl, args.seed + 31)).to(dev)
         lclock = KernelClock(torch)
-        for sb in (96, 128, 160, 192, 224, 256):
-            for _ in range(3):
-                lclock.run(f"copy_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="shfl"))
-                lclock.run(f"copy_naive_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="direct"))
-                lclock.run(f"gather_{sb}", (2 * sb + 4) * nl,
-                           lambda: T.gather(src, idx_l, dst, nl, sb, variant="tile"))
-                lclock.run(f"gather_naive_{sb}", (2 * sb + 4) * nl,
-                           lambda: T.gather(src, idx_l, dst, nl, sb, variant="direct"))
-        torch.cuda.synchronize()
+        for nl, sizes in ((1 << 24, (96, 128, 160, 192, 224, 256)), (1 << 22, (384, 512, 768, 1024))):
+            # Up to 4 GiB of rows = the size of src / dst; indices are their
+            # own permutation of [0, nl).
+            assert nl * max(sizes) <= src.numel() * 4 and nl * max(sizes) <= dst.numel() * 4
+            idx_l = torch.from_numpy(T.gen_permutation(nl, args.seed + 31)).to(dev)
+            for sb in sizes:
+                for _ in range(3):
+                    lclock.run(f"copy_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="shfl"))
+                    lclock.run(f"copy_naive_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="direct"))
+                    lclock.run(f"gather_{sb}", (2 * sb + 4) * nl,
+                               lambda: T.gather(src, idx_l, dst, nl, sb, variant="tile"))
+                    lclock.run(f"gather_naive_{sb}", (2 * sb + 4) * nl,
+                               lambda: T.gather(src, idx_l, dst, nl, sb, variant="direct"))
+            torch.cuda.synchronize()
+            del idx_l
         lclock.collect()
-        del idx_l
-        large = {"workload": "struct sizes 96..256 B (composite schedule at 96/160/192/224), N=2^24, "
-                             "gather through a seeded permutation of [0, 2^24)"}
+        large = {"workload": "struct sizes 96..1024 B (composite schedule at 96/160/192/224/384/768), "
+                             "N=2^24 up to 256 B and 2^22 above, gather through a seeded permutation of [0, N)"}
         for name, (launches, ms, nbytes) in lclock.totals.items():
             large[name + "_gbs"] = round(nbytes / launches / (ms / launches * 1e-3) / 1e9, 1)
     block = None

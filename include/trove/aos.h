@@ -211,7 +211,7 @@ __device__ __forceinline__ void tile_scatter_indexed(T* base, int index, int id,
 // Policy is trove::cached (default: ordinary accesses, as in the reference)
 // or trove::streaming (nontemporal accesses for data touched once).
 template <std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T>
-__device__ typename enable_if<detail::use_shfl<T, tile_size>::value, T>::type load_warp_contiguous(const T* src) {
+__device__ __forceinline__ typename enable_if<detail::use_shfl<T, tile_size>::value, T>::type load_warp_contiguous(const T* src) {
     constexpr int n = (int)tile_size;
     typedef detail::word_layout<T> L;
     const int lane = lane_id();
@@ -220,12 +220,12 @@ __device__ typename enable_if<detail::use_shfl<T, tile_size>::value, T>::type lo
     detail::tile_load_words<n, T, Policy>(src - id, id, v);
     detail::r2c_words<n, L::m, L::W>(v, id, lane & ~(n - 1));
     T out;
-    __builtin_memcpy(&out, v, sizeof(T));
+    detail::words_to_struct(v, out);
     return out;
 }
 
 template <std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T>
-__device__ typename enable_if<detail::use_direct<T, tile_size>::value, T>::type load_warp_contiguous(const T* src) {
+__device__ __forceinline__ typename enable_if<detail::use_direct<T, tile_size>::value, T>::type load_warp_contiguous(const T* src) {
     if constexpr (detail::use_divergent<T>::value) {
         typedef detail::word_layout<T> L;
         const typename L::U* p = reinterpret_cast<const typename L::U*>(src);
@@ -233,7 +233,7 @@ __device__ typename enable_if<detail::use_direct<T, tile_size>::value, T>::type 
 #pragma unroll
         for (int j = 0; j < L::m; ++j) w[j] = detail::load_word<Policy>(p + j);
         T out;
-        __builtin_memcpy(&out, w, sizeof(T));
+        detail::words_to_struct(w, out);
         return out;
     } else {
         return detail::divergent_load(src);
@@ -241,25 +241,25 @@ __device__ typename enable_if<detail::use_direct<T, tile_size>::value, T>::type 
 }
 
 template <std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T>
-__device__ typename enable_if<detail::use_shfl<T, tile_size>::value>::type store_warp_contiguous(const T& data,
+__device__ __forceinline__ typename enable_if<detail::use_shfl<T, tile_size>::value>::type store_warp_contiguous(const T& data,
                                                                                                  T* dest) {
     constexpr int n = (int)tile_size;
     typedef detail::word_layout<T> L;
     const int lane = lane_id();
     const int id = lane & (n - 1);
     uint32_t v[L::m * L::W];
-    __builtin_memcpy(v, &data, sizeof(T));
+    detail::struct_to_words(data, v);
     detail::c2r_words<n, L::m, L::W>(v, id, lane & ~(n - 1));
     detail::tile_store_words<n, T, Policy>(dest - id, id, v);
 }
 
 template <std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T>
-__device__ typename enable_if<detail::use_direct<T, tile_size>::value>::type store_warp_contiguous(const T& data,
+__device__ __forceinline__ typename enable_if<detail::use_direct<T, tile_size>::value>::type store_warp_contiguous(const T& data,
                                                                                                    T* dest) {
     if constexpr (detail::use_divergent<T>::value) {
         typedef detail::word_layout<T> L;
         typename L::U w[L::m];
-        __builtin_memcpy(w, &data, sizeof(T));
+        detail::struct_to_words(data, w);
         typename L::U* p = reinterpret_cast<typename L::U*>(dest);
 #pragma unroll
         for (int j = 0; j < L::m; ++j) detail::store_word<Policy>(p + j, w[j]);
@@ -271,7 +271,7 @@ __device__ typename enable_if<detail::use_direct<T, tile_size>::value>::type sto
 namespace detail {
 
 template <typename T, typename Tile>
-__device__ typename enable_if<use_shfl<T, Tile::size()>::value, T>::type load_dispatch(const T* src,
+__device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value, T>::type load_dispatch(const T* src,
                                                                                        const Tile& tile) {
     constexpr int n = Tile::size();
     typedef word_layout<T> L;
@@ -287,17 +287,17 @@ __device__ typename enable_if<use_shfl<T, Tile::size()>::value, T>::type load_di
     }
     r2c_words<n, L::m, L::W>(v, id, first);
     T out;
-    __builtin_memcpy(&out, v, sizeof(T));
+    words_to_struct(v, out);
     return out;
 }
 
 template <typename T, typename Tile>
-__device__ typename enable_if<use_direct<T, Tile::size()>::value, T>::type load_dispatch(const T* src, const Tile&) {
+__device__ __forceinline__ typename enable_if<use_direct<T, Tile::size()>::value, T>::type load_dispatch(const T* src, const Tile&) {
     return divergent_load(src);
 }
 
 template <typename T, typename Tile>
-__device__ typename enable_if<use_shfl<T, Tile::size()>::value>::type store_dispatch(const T& data, T* dest,
+__device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value>::type store_dispatch(const T& data, T* dest,
                                                                                      const Tile& tile) {
     constexpr int n = Tile::size();
     typedef word_layout<T> L;
@@ -306,7 +306,7 @@ __device__ typename enable_if<use_shfl<T, Tile::size()>::value>::type store_disp
     T* p0 = tile_first_pointer<n>(dest, first);
     const long long delta = (long long)(dest - p0);
     uint32_t v[L::m * L::W];
-    __builtin_memcpy(v, &data, sizeof(T));
+    struct_to_words(data, v);
     c2r_words<n, L::m, L::W>(v, id, first);
     if (tile_all<n>(delta == id)) {
         tile_store_words<n>(p0, id, v);
@@ -316,7 +316,7 @@ __device__ typename enable_if<use_shfl<T, Tile::size()>::value>::type store_disp
 }
 
 template <typename T, typename Tile>
-__device__ typename enable_if<use_direct<T, Tile::size()>::value>::type store_dispatch(const T& data, T* dest,
+__device__ __forceinline__ typename enable_if<use_direct<T, Tile::size()>::value>::type store_dispatch(const T& data, T* dest,
                                                                                        const Tile&) {
     divergent_store(data, dest);
 }
@@ -325,7 +325,7 @@ __device__ typename enable_if<use_direct<T, Tile::size()>::value>::type store_di
 
 // Load *src cooperatively with the widest converged tile around the lane.
 template <typename T>
-__device__ T load(const T* src) {
+__device__ __forceinline__ T load(const T* src) {
     if (warp_converged<WARP_SIZE>()) {
         return detail::load_dispatch(src, thread_tile<WARP_SIZE>());
     } else if (warp_converged<WARP_SIZE / 2>()) {
@@ -345,7 +345,7 @@ __device__ T load(const T* src) {
 
 // Store data to *dest cooperatively with the widest converged tile.
 template <typename T>
-__device__ void store(const T& data, T* dest) {
+__device__ __forceinline__ void store(const T& data, T* dest) {
     if (warp_converged<WARP_SIZE>()) {
         detail::store_dispatch(data, dest, thread_tile<WARP_SIZE>());
     } else if (warp_converged<WARP_SIZE / 2>()) {

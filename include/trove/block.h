@@ -12,28 +12,28 @@
 namespace trove {
 
 template <int s, std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T, typename I>
-__device__ trove::array<T, s> load_array_warp_contiguous(const T* src, const I& idx) {
+__device__ __forceinline__ trove::array<T, s> load_array_warp_contiguous(const T* src, const I& idx) {
     typedef trove::array<T, s> array_type;
     const array_type* src_ptr = (const array_type*)(src) + idx;
     return load_warp_contiguous<tile_size, Policy>(src_ptr);
 }
 
 template <int s, typename T, typename I>
-__device__ trove::array<T, s> load_array(const T* src, const I& idx) {
+__device__ __forceinline__ trove::array<T, s> load_array(const T* src, const I& idx) {
     typedef trove::array<T, s> array_type;
     const array_type* src_ptr = (const array_type*)(src) + idx;
     return *src_ptr;
 }
 
 template <int s, std::size_t tile_size = WARP_SIZE, typename Policy = cached, typename T, typename I>
-__device__ void store_array_warp_contiguous(T* dest, const I& idx, const trove::array<T, s>& src) {
+__device__ __forceinline__ void store_array_warp_contiguous(T* dest, const I& idx, const trove::array<T, s>& src) {
     typedef trove::array<T, s> array_type;
     array_type* dest_ptr = (array_type*)(dest) + idx;
     store_warp_contiguous<tile_size, Policy>(src, dest_ptr);
 }
 
 template <int s, typename T, typename I>
-__device__ void store_array(T* dest, const I& idx, const trove::array<T, s>& src) {
+__device__ __forceinline__ void store_array(T* dest, const I& idx, const trove::array<T, s>& src) {
     typedef trove::array<T, s> array_type;
     array_type* dest_ptr = (array_type*)(dest) + idx;
     *dest_ptr = src;

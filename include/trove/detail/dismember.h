@@ -87,18 +87,41 @@ __device__ __forceinline__ void store_word(U* p, const U& u) {
     }
 }
 
+// Struct <-> words, one memcpy per word: a single memcpy of a struct of
+// several hundred bytes is not split into registers and lands in scratch
+// memory, while word-sized copies stay in VGPRs.
+template <typename U, typename T>
+__host__ __device__ __forceinline__ void struct_to_words(const T& in, U* w) {
+    static_assert(sizeof(T) % sizeof(U) == 0, "struct is not a whole number of words");
+#pragma unroll
+    for (int j = 0; j < (int)(sizeof(T) / sizeof(U)); ++j)
+        __builtin_memcpy(&w[j], reinterpret_cast<const char*>(&in) + j * sizeof(U), sizeof(U));
+}
+
+template <typename U, typename T>
+__host__ __device__ __forceinline__ void words_to_struct(const U* w, T& out) {
+    static_assert(sizeof(T) % sizeof(U) == 0, "struct is not a whole number of words");
+#pragma unroll
+    for (int j = 0; j < (int)(sizeof(T) / sizeof(U)); ++j)
+        __builtin_memcpy(reinterpret_cast<char*>(&out) + j * sizeof(U), &w[j], sizeof(U));
+}
+
 template <typename U, typename T>
 __host__ __device__ array<U, aliased_size<T, U>::value> lyse(const T& in) {
     array<U, aliased_size<T, U>::value> out;
     static_assert(sizeof(out) == sizeof(T), "word array must alias the struct");
-    __builtin_memcpy(&out, &in, sizeof(T));
+    U w[aliased_size<T, U>::value];
+    struct_to_words(in, w);
+    words_to_struct(w, out);
     return out;
 }
 
 template <typename T, typename U>
 __host__ __device__ T fuse(const array<U, aliased_size<T, U>::value>& in) {
+    U w[aliased_size<T, U>::value];
+    struct_to_words(in, w);
     T out;
-    __builtin_memcpy(&out, &in, sizeof(T));
+    words_to_struct(w, out);
     return out;
 }
 

@@ -15,16 +15,20 @@ struct use_divergent {
     static const bool value = (sizeof(T) % 4) == 0;
 };
 
+// Each word is copied into (out of) the struct on its own: one memcpy of a
+// whole struct of several hundred bytes is not split into registers and
+// lands in scratch memory.
 template <typename T>
 __device__ __forceinline__ typename enable_if<use_divergent<T>::value, T>::type divergent_load(const T* src) {
     typedef typename dismember_type<T>::type U;
     constexpr int m = aliased_size<T, U>::value;
     const U* p = reinterpret_cast<const U*>(src);
-    U w[m];
-#pragma unroll
-    for (int j = 0; j < m; ++j) w[j] = p[j];
     T out;
-    __builtin_memcpy(&out, w, sizeof(T));
+#pragma unroll
+    for (int j = 0; j < m; ++j) {
+        const U u = p[j];
+        __builtin_memcpy(reinterpret_cast<char*>(&out) + j * sizeof(U), &u, sizeof(U));
+    }
     return out;
 }
 
@@ -37,11 +41,13 @@ template <typename T>
 __device__ __forceinline__ typename enable_if<use_divergent<T>::value>::type divergent_store(const T& data, T* dest) {
     typedef typename dismember_type<T>::type U;
     constexpr int m = aliased_size<T, U>::value;
-    U w[m];
-    __builtin_memcpy(w, &data, sizeof(T));
     U* p = reinterpret_cast<U*>(dest);
 #pragma unroll
-    for (int j = 0; j < m; ++j) p[j] = w[j];
+    for (int j = 0; j < m; ++j) {
+        U u;
+        __builtin_memcpy(&u, reinterpret_cast<const char*>(&data) + j * sizeof(U), sizeof(U));
+        p[j] = u;
+    }
 }
 
 template <typename T>

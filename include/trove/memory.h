@@ -12,13 +12,13 @@
 namespace trove {
 
 template <typename Array>
-__host__ __device__ void warp_store(const Array& t, typename Array::head_type* ptr, int offset, int stride) {
+__host__ __device__ __forceinline__ void warp_store(const Array& t, typename Array::head_type* ptr, int offset, int stride) {
     ptr[offset] = t.head;
     if constexpr (Array::size > 1) warp_store(t.tail, ptr, offset + stride, stride);
 }
 
 template <typename Array>
-__host__ __device__ Array warp_load(const typename Array::head_type* ptr, int offset, int stride) {
+__host__ __device__ __forceinline__ Array warp_load(const typename Array::head_type* ptr, int offset, int stride) {
     if constexpr (Array::size == 1) {
         return Array(ptr[offset]);
     } else {
@@ -27,7 +27,7 @@ __host__ __device__ Array warp_load(const typename Array::head_type* ptr, int of
 }
 
 template <typename Array>
-__host__ __device__ Array warp_load(const volatile typename Array::head_type* ptr, int offset, int stride) {
+__host__ __device__ __forceinline__ Array warp_load(const volatile typename Array::head_type* ptr, int offset, int stride) {
     if constexpr (Array::size == 1) {
         return Array(ptr[offset]);
     } else {
@@ -36,13 +36,13 @@ __host__ __device__ Array warp_load(const volatile typename Array::head_type* pt
 }
 
 template <typename Array>
-__host__ __device__ void uncoalesced_store(const Array& t, typename Array::head_type* ptr, int stride = 1) {
+__host__ __device__ __forceinline__ void uncoalesced_store(const Array& t, typename Array::head_type* ptr, int stride = 1) {
     ptr[0] = t.head;
     if constexpr (Array::size > 1) uncoalesced_store(t.tail, ptr + 1, stride);
 }
 
 template <typename Array>
-__host__ __device__ void uncoalesced_store(const Array& t, volatile typename Array::head_type* ptr,
+__host__ __device__ __forceinline__ void uncoalesced_store(const Array& t, volatile typename Array::head_type* ptr,
                                            int stride = 1) {
     ptr[0] = t.head;
     if constexpr (Array::size > 1) uncoalesced_store(t.tail, ptr + 1, stride);

@@ -17,14 +17,14 @@ struct coalesced_ref {
     T* m_ptr;
     __device__ explicit coalesced_ref(T* ptr) : m_ptr(ptr) {}
 
-    __device__ operator T() { return trove::load(m_ptr); }
+    __device__ __forceinline__ operator T() { return trove::load(m_ptr); }
 
-    __device__ coalesced_ref& operator=(const T& data) {
+    __device__ __forceinline__ coalesced_ref& operator=(const T& data) {
         trove::store(data, m_ptr);
         return *this;
     }
 
-    __device__ coalesced_ref& operator=(const coalesced_ref& other) {
+    __device__ __forceinline__ coalesced_ref& operator=(const coalesced_ref& other) {
         const T data = trove::load(other.m_ptr);
         trove::store(data, m_ptr);
         return *this;

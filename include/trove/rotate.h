@@ -48,7 +48,7 @@ __host__ __device__ __forceinline__ void rotate_words_left(uint32_t (&v)[m * W],
 }  // namespace detail
 
 template <typename T, int i>
-__host__ __device__ array<T, i> rotate(const array<T, i>& t, int a) {
+__host__ __device__ __forceinline__ array<T, i> rotate(const array<T, i>& t, int a) {
     static_assert(sizeof(array<T, i>) == i * sizeof(T), "array must be densely packed");
     if constexpr (i == 1 || sizeof(T) % 4 != 0) {
         // Generic element type: select element-wise.

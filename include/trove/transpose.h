@@ -99,7 +99,7 @@ struct array_words {
 }  // namespace detail
 
 template <typename Tile, int i>
-__device__ void c2r_compute_indices(array<int, i>& indices, int& rotation, const Tile& tile) {
+__device__ __forceinline__ void c2r_compute_indices(array<int, i>& indices, int& rotation, const Tile& tile) {
     const int id = tile.id();
     int p[i];
     detail::partner_table<Tile::size(), i>(id, p);
@@ -108,7 +108,7 @@ __device__ void c2r_compute_indices(array<int, i>& indices, int& rotation, const
 }
 
 template <typename Tile, int i>
-__device__ void r2c_compute_indices(array<int, i>& indices, int& rotation, const Tile& tile) {
+__device__ __forceinline__ void r2c_compute_indices(array<int, i>& indices, int& rotation, const Tile& tile) {
     const int id = tile.id();
     int p[i];
     detail::partner_table<Tile::size(), i>(id, p);
@@ -117,7 +117,7 @@ __device__ void r2c_compute_indices(array<int, i>& indices, int& rotation, const
 }
 
 template <typename Tile, typename T, int i>
-__device__ void c2r_warp_transpose(array<T, i>& src, const array<int, i>& indices, int rotation, const Tile& tile) {
+__device__ __forceinline__ void c2r_warp_transpose(array<T, i>& src, const array<int, i>& indices, int rotation, const Tile& tile) {
     constexpr int W = detail::array_words<T, i>::W;
     uint32_t v[i * W];
     int p[i];
@@ -128,7 +128,7 @@ __device__ void c2r_warp_transpose(array<T, i>& src, const array<int, i>& indice
 }
 
 template <typename Tile, typename T, int i>
-__device__ void c2r_warp_transpose(array<T, i>& src, const Tile& tile) {
+__device__ __forceinline__ void c2r_warp_transpose(array<T, i>& src, const Tile& tile) {
     constexpr int W = detail::array_words<T, i>::W;
     uint32_t v[i * W];
     __builtin_memcpy(v, &src, sizeof(v));
@@ -137,7 +137,7 @@ __device__ void c2r_warp_transpose(array<T, i>& src, const Tile& tile) {
 }
 
 template <typename Tile, typename T, int i>
-__device__ void r2c_warp_transpose(array<T, i>& src, const array<int, i>& indices, int rotation, const Tile& tile) {
+__device__ __forceinline__ void r2c_warp_transpose(array<T, i>& src, const array<int, i>& indices, int rotation, const Tile& tile) {
     constexpr int W = detail::array_words<T, i>::W;
     uint32_t v[i * W];
     int p[i];
@@ -148,7 +148,7 @@ __device__ void r2c_warp_transpose(array<T, i>& src, const array<int, i>& indice
 }
 
 template <typename Tile, typename T, int i>
-__device__ void r2c_warp_transpose(array<T, i>& src, const Tile& tile) {
+__device__ __forceinline__ void r2c_warp_transpose(array<T, i>& src, const Tile& tile) {
     constexpr int W = detail::array_words<T, i>::W;
     uint32_t v[i * W];
     __builtin_memcpy(v, &src, sizeof(v));

@@ -84,11 +84,13 @@ int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int
 
 /* dst[i] = counting_array(i * k) (k = struct_bytes / 4 words): the store
  * benchmark of tests/benchmark.cu:52-72 (SHFL: store_warp_contiguous;
- * DIRECT: plain stores). struct_bytes must be a multiple of 4. */
+ * DIRECT: plain stores). struct_bytes must be a multiple of 4 and at most
+ * 256 (else TROVE_ERR_SIZE). */
 int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, void* stream);
 
 /* out[i] = sum of the 32-bit words of src[i] (wrapping): the load benchmark
- * of tests/benchmark.cu:74-88. struct_bytes must be a multiple of 4. */
+ * of tests/benchmark.cu:74-88. struct_bytes must be a multiple of 4 and at
+ * most 256 (else TROVE_ERR_SIZE). */
 int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, int variant, void* stream);
 
 /* Block interface copy over n_elems 32-bit elements, s per lane

@@ -37,7 +37,7 @@ __device__ uint32_t decode(const T& t) {
 // explicit-index R2C must produce r*M + j; then the self-contained C2R must
 // restore the input.
 template <int N, int M, typename T>
-__global__ void k_transpose(uint32_t* r2c_out, uint32_t* c2r_out) {
+__global__ __launch_bounds__(64) void k_transpose(uint32_t* r2c_out, uint32_t* c2r_out) {
     trove::thread_tile<N> tile;
     const int r = tile.id();
     const int lane = threadIdx.x;
@@ -58,7 +58,7 @@ __global__ void k_transpose(uint32_t* r2c_out, uint32_t* c2r_out) {
 }
 
 template <int M>
-__global__ void k_rotate(uint32_t* out) {
+__global__ __launch_bounds__(64) void k_rotate(uint32_t* out) {
     const int lane = threadIdx.x;
     int lim = 1;
     while (lim < M) lim <<= 1;
@@ -71,7 +71,7 @@ __global__ void k_rotate(uint32_t* out) {
     for (int j = 0; j < M; ++j) out[lane * M + j] = t[j];
 }
 
-__global__ void k_converged(uint64_t mask, int32_t* out) {
+__global__ __launch_bounds__(64) void k_converged(uint64_t mask, int32_t* out) {
     const int lane = threadIdx.x;
     for (int k = 0; k < 6; ++k) out[lane * 6 + k] = -1;
     if ((mask >> lane) & 1ull) {
@@ -85,7 +85,7 @@ __global__ void k_converged(uint64_t mask, int32_t* out) {
 }
 
 template <int N>
-__global__ void k_shfl(const int* src_rank, uint32_t* out) {
+__global__ __launch_bounds__(64) void k_shfl(const int* src_rank, uint32_t* out) {
     trove::thread_tile<N> tile;
     const int lane = threadIdx.x;
     const uint64_t v = (uint64_t)lane * 0x100000001ull + 7;
@@ -97,7 +97,7 @@ __global__ void k_shfl(const int* src_rank, uint32_t* out) {
 // Active lanes (mask) copy src[perm[lane]] to dst[lane] (gather) or
 // src[lane] to dst[perm[lane]] (scatter) through trove::load / trove::store.
 template <int S>
-__global__ void k_cascade(uint64_t mask, int scatter, const trove::array<uint32_t, S / 4>* src, const int* perm,
+__global__ __launch_bounds__(64) void k_cascade(uint64_t mask, int scatter, const trove::array<uint32_t, S / 4>* src, const int* perm,
                           trove::array<uint32_t, S / 4>* dst) {
     const int lane = threadIdx.x;
     if ((mask >> lane) & 1ull) {
@@ -132,6 +132,7 @@ int devtest_transpose(int n, int m, int word_bytes, uint32_t* r2c_out, uint32_t*
     TT(64, 2, uint64_t) TT(64, 3, uint64_t) TT(64, 6, uint64_t) TT(64, 12, uint64_t)
     TT(64, 2, w16) TT(64, 5, w16) TT(64, 6, w16) TT(64, 16, w16) TT(32, 6, w16)
     TT(64, 24, uint32_t) TT(64, 33, uint32_t) TT(64, 64, uint32_t)
+    TT(64, 24, w16) TT(64, 32, w16) TT(32, 32, w16) TT(64, 48, w16) TT(64, 64, w16)
 #undef TT_M
 #undef TT
     return -1;
@@ -171,7 +172,7 @@ int devtest_cascade(int s, uint64_t mask, int scatter, const void* src, const in
                            (const trove::array<uint32_t, S / 4>*)src, perm, (trove::array<uint32_t, S / 4>*)dst); \
         return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                                        \
     }
-    CS(8) CS(12) CS(20) CS(32) CS(48) CS(60) CS(96)
+    CS(8) CS(12) CS(20) CS(32) CS(48) CS(60) CS(96) CS(384) CS(512) CS(1024)
 #undef CS
     return -1;
 }

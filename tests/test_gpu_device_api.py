@@ -43,6 +43,7 @@ TRANSPOSE_CASES = (
     + [(8, m, 4) for m in (2, 3, 5, 6, 8)] + [(4, 2, 4), (4, 3, 4), (4, 4, 4), (2, 2, 4)]
     + [(64, m, 8) for m in (2, 3, 6, 12)] + [(64, m, 16) for m in (2, 5, 6, 16)] + [(32, 6, 16)]
     + [(64, 24, 4), (64, 33, 4), (64, 64, 4)]
+    + [(64, 24, 16), (64, 32, 16), (32, 32, 16), (64, 48, 16), (64, 64, 16)]
 )
 
 
@@ -110,7 +111,7 @@ def test_tile_shfl(dt, n):
         assert got[lane, 0] == v & 0xFFFFFFFF and got[lane, 1] == v >> 32, (n, lane)
 
 
-@pytest.mark.parametrize("s", [8, 12, 20, 32, 48, 60, 96])
+@pytest.mark.parametrize("s", [8, 12, 20, 32, 48, 60, 96, 384, 512, 1024])
 def test_cascade_under_masks(dt, s):
     rng = np.random.default_rng(s)
     k = s // 4

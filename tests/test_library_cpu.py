@@ -37,9 +37,30 @@ def test_struct_size_table():
     L = T.lib()
     for b in T.STRUCT_SIZES:
         assert L.trove_struct_size_supported(b) == 1
-    for b in (0, 1, 3, 68, 100, 512):
+    for b in (0, 1, 3, 68, 100, 320, 1028, 2048):
         assert L.trove_struct_size_supported(b) == 0
-    assert [L.trove_struct_word_bytes(b) for b in (4, 8, 12, 16, 20, 24, 32, 6, 2)] == [4, 8, 4, 16, 4, 8, 16, 1, 1]
+    assert [L.trove_struct_word_bytes(b) for b in (4, 8, 12, 16, 20, 24, 32, 6, 2, 384, 1024)] == \
+        [4, 8, 4, 16, 4, 8, 16, 1, 1, 16, 16]
+
+
+def test_size_shards_match_build():
+    """build.py compiles one object per size shard that csrc/common.h defines."""
+    import re
+    from trove_amd import build
+    text = open(os.path.join(build.CSRC, "common.h")).read()
+    shards = int(re.search(r"#define TROVE_SIZE_SHARDS (\d+)", text).group(1))
+    assert shards == build.SIZE_SHARDS
+    sizes = sorted(int(x) for k in range(shards)
+                   for x in re.findall(r"X\((\d+)\)", re.search(rf"#define TROVE_SIZES_SHARD{k}\(X\)(.*)", text).group(1)))
+    assert sizes == sorted(T.STRUCT_SIZES)
+
+
+def test_bench_kernels_limited_to_256_bytes():
+    L = T.lib()
+    fake = 0x10000  # never dereferenced: the size check returns before any launch
+    for b in (384, 512, 768, 1024):
+        assert L.trove_store_counting(fake, 100, b, 1, None) == T.ERR_SIZE
+        assert L.trove_load_sum(fake, fake, 100, b, 1, None) == T.ERR_SIZE
 
 
 def test_validation_before_launch():

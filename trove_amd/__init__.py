@@ -41,7 +41,7 @@ VARIANT_TILE = 3
 VARIANTS = {"auto": VARIANT_AUTO, "shfl": VARIANT_SHFL, "direct": VARIANT_DIRECT, "tile": VARIANT_TILE}
 
 # Struct sizes compiled into the library (bytes).
-STRUCT_SIZES = tuple(range(4, 65, 4)) + (96, 128, 160, 192, 224, 256, 2, 6)
+STRUCT_SIZES = tuple(range(4, 65, 4)) + (96, 128, 160, 192, 224, 256, 384, 512, 768, 1024, 2, 6)
 
 
 class TroveUnavailable(RuntimeError):

@@ -42,7 +42,7 @@ HOST_FLAGS = ["-std=c++17", "-O3", "-fPIC", "-I" + INCLUDE, "-Wall"]
 # Translation units instantiated per struct size are compiled once per size
 # shard (-DTROVE_SHARD=k) so that the shards build in parallel.
 SHARDED = {"copy.hip", "gather.hip"}
-SIZE_SHARDS = 6
+SIZE_SHARDS = 10  # TROVE_SIZE_SHARDS in csrc/common.h
 
 
 def _sources():

@@ -23,21 +23,31 @@ using blob_t = typename std::conditional<B % 4 == 0, trove::array<uint32_t, (B %
                                          trove::array<uint8_t, B> >::type;
 
 // Struct sizes compiled into the C ABI: the 4..64-byte sweep, larger sizes
-// that reach the composite schedule (96, 160, 192, 224 B: 6, 10, 12, 14
-// 16-byte words) and two sizes that are not a multiple of 4. They are split
-// into shards so that the kernel translation units can be compiled in
-// parallel: a unit built with -DTROVE_SHARD=k instantiates shard k only
-// (without it, all shards).
+// that reach the composite schedule (96, 160, 192, 224, 384, 768 B: 6, 10,
+// 12, 14, 24, 48 16-byte words), power-of-two sizes up to the whole-wavefront
+// limit m = 64 (1 KiB) and two sizes that are not a multiple of 4. They are
+// split into shards so that the kernel translation units can be compiled in
+// parallel (one shard per size above 256 B, whose kernels hold up to 256
+// words per lane and compile slowest): a unit built with -DTROVE_SHARD=k
+// instantiates shard k only (without it, all shards).
 #define TROVE_SIZES_SHARD0(X) X(4) X(8) X(12) X(16) X(20) X(224)
 #define TROVE_SIZES_SHARD1(X) X(24) X(28) X(32) X(36) X(256)
 #define TROVE_SIZES_SHARD2(X) X(40) X(44) X(48) X(2) X(192)
 #define TROVE_SIZES_SHARD3(X) X(52) X(56) X(6) X(160)
 #define TROVE_SIZES_SHARD4(X) X(60) X(64) X(128)
 #define TROVE_SIZES_SHARD5(X) X(96)
-#define TROVE_SIZE_SHARDS 6
+#define TROVE_SIZES_SHARD6(X) X(384)
+#define TROVE_SIZES_SHARD7(X) X(512)
+#define TROVE_SIZES_SHARD8(X) X(768)
+#define TROVE_SIZES_SHARD9(X) X(1024)
+#define TROVE_SIZE_SHARDS 10
 #define TROVE_FOR_EACH_SIZE(X)                                                                                 \
     TROVE_SIZES_SHARD0(X) TROVE_SIZES_SHARD1(X) TROVE_SIZES_SHARD2(X) TROVE_SIZES_SHARD3(X) TROVE_SIZES_SHARD4(X) \
-        TROVE_SIZES_SHARD5(X)
+        TROVE_SIZES_SHARD5(X) TROVE_SIZES_SHARD6(X) TROVE_SIZES_SHARD7(X) TROVE_SIZES_SHARD8(X) TROVE_SIZES_SHARD9(X)
+
+// The store / load-sum benchmark kernels of the reference harness are
+// compiled up to this struct size.
+constexpr int kMaxBenchStructBytes = 256;
 
 #if !defined(TROVE_SHARD)
 #define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_FOR_EACH_SIZE(X)
@@ -53,6 +63,14 @@ using blob_t = typename std::conditional<B % 4 == 0, trove::array<uint32_t, (B %
 #define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD4(X)
 #elif TROVE_SHARD == 5
 #define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD5(X)
+#elif TROVE_SHARD == 6
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD6(X)
+#elif TROVE_SHARD == 7
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD7(X)
+#elif TROVE_SHARD == 8
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD8(X)
+#elif TROVE_SHARD == 9
+#define TROVE_FOR_EACH_SIZE_IN_SHARD(X) TROVE_SIZES_SHARD9(X)
 #endif
 
 // Global wavefront index and count for B-thread work-groups (scalar: the

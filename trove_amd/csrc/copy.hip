@@ -47,10 +47,6 @@ __global__ __launch_bounds__(B) void k_copy_shfl(const T* __restrict__ src, T* _
                 }
             }
         }
-        if (wave_index<B>() == 0 && (n & 63)) {
-            const int64_t i = (groups << 6) + lane;
-            if (i < n) trove::store(trove::load(src + i), dst + i);
-        }
         return;
     }
     const int64_t step = wave_count<B>() * UNROLL;
@@ -68,11 +64,15 @@ __global__ __launch_bounds__(B) void k_copy_shfl(const T* __restrict__ src, T* _
         const T v = trove::load_warp_contiguous<TILE, trove::streaming>(src + (g << 6) + lane);
         trove::store_warp_contiguous<TILE, trove::streaming>(v, dst + (g << 6) + lane);
     }
-    // Partial trailing group: the active lanes go through the tile cascade.
-    if (wave_index<B>() == 0 && (n & 63)) {
-        const int64_t i = (groups << 6) + lane;
-        if (i < n) trove::store(trove::load(src + i), dst + i);
-    }
+}
+
+// Partial trailing group (n % 64 structs), one wavefront: the active lanes go
+// through the tile cascade. A kernel of its own, so that the streaming kernel
+// does not carry the cascade's registers (up to 512 per lane at 1 KiB).
+template <typename T>
+__global__ __launch_bounds__(64) void k_copy_tail(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
+    const int64_t i = ((n >> 6) << 6) + trove::lane_id();
+    if (i < n) trove::store(trove::load(src + i), dst + i);
 }
 
 template <typename T>
@@ -87,9 +87,17 @@ static int launch_copy_shfl(const T* src, T* dst, int64_t n, hipStream_t s) {
     const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
     const int64_t blocks = grid_for(units, C::block / 64);
     const size_t lds = stream_lds_reservation((int64_t)C::unroll * 64 * (int64_t)sizeof(T) * (C::block / 64));
-    hipLaunchKernelGGL((k_copy_shfl<T, TILE, C::unroll, C::block, C::xcd_map>), dim3((unsigned)blocks),
-                       dim3(C::block), lds, s, src, dst, n);
-    return launch_status();
+    if (n >= 64) {
+        hipLaunchKernelGGL((k_copy_shfl<T, TILE, C::unroll, C::block, C::xcd_map>), dim3((unsigned)blocks),
+                           dim3(C::block), lds, s, src, dst, n);
+        const int rc = launch_status();
+        if (rc != TROVE_OK) return rc;
+    }
+    if (n & 63) {
+        hipLaunchKernelGGL((k_copy_tail<T>), dim3(1), dim3(64), 0, s, src, dst, n);
+        return launch_status();
+    }
+    return TROVE_OK;
 }
 
 template <int B>
@@ -187,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_loadsum_direct(const T* __restrict__
 
 template <int B>
 int counting_impl(void* dst_, int64_t n, int variant, hipStream_t s) {
-    if constexpr (B % 4 != 0) {
+    if constexpr (B % 4 != 0 || B > kMaxBenchStructBytes) {
         return TROVE_ERR_SIZE;
     } else {
         typedef blob_t<B> T;
@@ -211,7 +219,7 @@ int counting_impl(void* dst_, int64_t n, int variant, hipStream_t s) {
 
 template <int B>
 int loadsum_impl(const void* src_, uint32_t* out, int64_t n, int variant, hipStream_t s) {
-    if constexpr (B % 4 != 0) {
+    if constexpr (B % 4 != 0 || B > kMaxBenchStructBytes) {
         return TROVE_ERR_SIZE;
     } else {
         typedef blob_t<B> T;
