@@ -36,7 +36,7 @@ def log(msg: str) -> None:
 def parse_args():
     p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--n-log2", type=int, default=26, help="structs per struct size (log2)")
     p.add_argument("--k", type=str, default="1-16", help="struct sizes in 32-bit words, e.g. 1-16 or 5,16")
@@ -72,6 +72,7 @@ class KernelClock:
         self.torch = torch
         self.pending = []  # (name, bytes, start, end)
         self.totals = {}   # name -> [launches, ms, bytes]
+        self.samples = {}  # name -> [ms per launch]
         self.seen = set() if seen is None else seen
 
     def run(self, name, nbytes, fn):
@@ -88,9 +89,11 @@ class KernelClock:
     def collect(self):
         for name, nbytes, s, e in self.pending:
             rec = self.totals.setdefault(name, [0, 0.0, 0])
+            ms = s.elapsed_time(e)
             rec[0] += 1
-            rec[1] += s.elapsed_time(e)
+            rec[1] += ms
             rec[2] += nbytes
+            self.samples.setdefault(name, []).append(ms)
         self.pending.clear()
 
 
@@ -246,7 +249,8 @@ def bench_served(args, torch, T):
     """Served gather: table of 2^24 rows x 32 B; requests of log-uniform
     2^8..2^18 indices drawn Zipf(1.1) over the table, indices device-resident;
     a saturation run (all requests queued at t = 0) fixes the request rate,
-    then Poisson arrivals at 50% and 80% of it; 4 GiB output budget."""
+    then Poisson arrivals at 25%, 50%, 80% and 95% of it; 4 GiB output
+    budget."""
     import numpy as np
 
     dev = torch.device("cuda:0")
@@ -292,7 +296,7 @@ def bench_served(args, torch, T):
                        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"]},
     }
     rng = np.random.default_rng(args.seed + 13)
-    for frac in (0.5, 0.8):
+    for frac in (0.25, 0.5, 0.8, 0.95):
         rate = frac * sat_rate
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=R))
         lat, wall, st = run(arrivals)
@@ -387,7 +391,10 @@ def main():
     per = {}
     for name, (launches, ms, nbytes) in clock.totals.items():
         avg_ms = ms / launches
-        per[name] = {"avg_ms": round(avg_ms, 4), "gbs": round(nbytes / launches / (avg_ms * 1e-3) / 1e9, 1)}
+        med_ms = statistics.median(clock.samples[name])
+        per[name] = {"avg_ms": round(avg_ms, 4), "median_ms": round(med_ms, 4),
+                     "gbs": round(nbytes / launches / (avg_ms * 1e-3) / 1e9, 1),
+                     "median_gbs": round(nbytes / launches / (med_ms * 1e-3) / 1e9, 1)}
     dominant = max(clock.totals.items(), key=lambda kv: kv[1][1])
     dname, (dl, dms, dbytes) = dominant
     d_gbs = dbytes / dl / (dms / dl * 1e-3) / 1e9
@@ -428,7 +435,10 @@ def main():
         row = {"k": k, "bytes": 4 * k,
                "copy_gbs": per[f"copy_shfl_k{k}"]["gbs"],
                "gather_tile_gbs": per[f"gather_tile_k{k}"]["gbs"],
-               "gather_coalesced_ptr_gbs": per[f"gather_shfl_k{k}"]["gbs"]}
+               "gather_coalesced_ptr_gbs": per[f"gather_shfl_k{k}"]["gbs"],
+               "median_gbs": {"copy": per[f"copy_shfl_k{k}"]["median_gbs"],
+                              "gather_tile": per[f"gather_tile_k{k}"]["median_gbs"],
+                              "gather_coalesced_ptr": per[f"gather_shfl_k{k}"]["median_gbs"]}}
         if naive:
             cd, gd = naive[f"copy_direct_k{k}"]["gbs"], naive[f"gather_direct_k{k}"]["gbs"]
             row.update({"copy_naive_gbs": cd, "gather_naive_gbs": gd,
@@ -496,12 +506,23 @@ def main():
         s_src = torch.from_numpy(words.view(np.int32)).to(dev)
         s_dst = torch.zeros_like(s_src)
         t_gpu = statistics.median(timed_runs(torch, lambda: T.copy(s_src, s_dst, ns, ss, variant="shfl"), 20))
+        # Flushed: 1 GiB of writes between launches evicts the 42 MB working
+        # set from the Infinity Cache; only the copy is timed.
+        flush = torch.empty(1 << 28, dtype=torch.int32, device=dev)
+        cold = []
+        for r in range(20):
+            T.fill_words_device(flush, flush.numel(), r)
+            cold.extend(timed_runs(torch, lambda: T.copy(s_src, s_dst, ns, ss, variant="shfl"), 1))
+        t_cold = statistics.median(cold)
+        del flush
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         want = oracle.copy(words.view(np.uint8), ns, ss)
         t_cpu = time.perf_counter() - t0
-        small = {"workload": "contiguous copy, 20-byte structs, N=2^20 (warm: fits the Infinity Cache)",
+        small = {"workload": "contiguous copy, 20-byte structs, N=2^20; warm (fits the Infinity Cache) and "
+                             "flushed (1 GiB written between launches); median of 20 launches each",
                  "gpu_gbs": round(2 * ss * ns / (t_gpu * 1e-3) / 1e9, 1),
+                 "gpu_flushed_gbs": round(2 * ss * ns / (t_cold * 1e-3) / 1e9, 1),
                  "cpu_scalar_gbs": round(2 * ss * ns / t_cpu / 1e9, 2),
                  "bit_exact": bool(np.array_equal(s_dst.cpu().numpy().view(np.uint8), want))}
         del s_src, s_dst
