@@ -123,3 +123,12 @@ def test_reference_style_kernels_compile(tmp_path):
                            os.path.join(ROOT, "tests", "host", "dropin_usage.hip"), "-o", str(tmp_path / "d.o")],
                           capture_output=True, text=True)
     assert proc.returncode == 0, proc.stderr[-4000:]
+
+
+def test_host_drivers_reject_bad_arguments_before_any_gpu_call():
+    host = os.path.join(ROOT, "tests", "host")
+    for args in (["trove_benchmark", "--n", "0"], ["trove_benchmark", "--upper", "17"], ["trove_block", "--s", "17"]):
+        exe = os.path.join(host, args[0])
+        assert os.path.exists(exe), f"{exe} not built"
+        p = subprocess.run([exe, *args[1:]], capture_output=True, text=True, timeout=60)
+        assert p.returncode == 2, (args, p.stdout, p.stderr)

@@ -115,6 +115,32 @@ def build_devtests() -> str:
     return DEVTESTS_LIB
 
 
+HOST_DRIVERS = ("trove_benchmark", "trove_block")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def build_host_drivers() -> list:
+    """Compiles the C++ host drivers (tests/host/trove_benchmark.cpp,
+    trove_block.cpp): host code against the C ABI and the HIP runtime, linked
+    to the in-tree libtrove_amd.so (call after build())."""
+    out = []
+    for name in HOST_DRIVERS:
+        exe = os.path.join(ROOT, "tests", "host", name)
+        tmp = exe + ".tmp"
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + INCLUDE,
+               "-I" + os.path.join(ROCM, "include"), os.path.join(ROOT, "tests", "host", name + ".cpp"),
+               "-L" + PKG_DIR, "-ltrove_amd", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+               "-Wl,-rpath,$ORIGIN/../../trove_amd", "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", tmp]
+        rc, line, text = _compile(cmd)
+        if rc != 0:
+            print(line)
+            print(text)
+            raise RuntimeError(f"{name} failed to compile")
+        os.replace(tmp, exe)
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     path = build(verbose="-v" in sys.argv)
     print(path)
