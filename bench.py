@@ -115,7 +115,7 @@ def cpu_baseline(ks, n_log2, seed):
         oracle.copy(view, n, s, omp=True)          # warm-up (page faults)
         oracle.gather(view, idx, s, omp=True)
         times = []
-        for _ in range(3):
+        for _ in range(5):
             t0 = time.perf_counter()
             oracle.copy(view, n, s, omp=True)
             oracle.gather(view, idx, s, omp=True)
@@ -129,7 +129,8 @@ def cpu_baseline(ks, n_log2, seed):
         "cores": threads,
         "kind": "port",
         "sample": (f"oracle OpenMP copy + gather (memcpy per struct), k={ks[0]}..{ks[-1]} words, "
-                   f"N=2^{n_log2} structs per size, median of 3, {threads} threads"),
+                   f"N=2^{n_log2} structs per size, median of 5 after a warm-up, {threads} OpenMP threads "
+                   f"(nproc {os.cpu_count()})"),
     }
 
 
