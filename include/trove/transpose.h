@@ -69,6 +69,66 @@ __device__ __forceinline__ void c2r_words(uint32_t (&v)[m * W], int id, int firs
     for (int i = 0; i < m * W; ++i) v[i] = t[i];
 }
 
+// Square tiles (m == n, a power of two): the transposition is the plain
+// n x n transpose of (lane, slot) -- column side lane a slot b holds word
+// a + n*b, which the row side wants in lane b slot a -- and is its own
+// inverse. It runs as log2(n) butterfly stages without LDS or runtime
+// rotations: stage d (= 1, 2, 4, ... n/2) exchanges, for every slot pair
+// (b, b + d) with bit d of b clear, the slot b of lanes with bit d set against
+// the slot b + d of their partner lane l ^ d. Lane distance 32 is one
+// v_permlane32_swap per slot pair and 16 one v_permlane16_swap; 8 and 4 are
+// one DPP move per slot (row_ror, writing only the lanes of the DPP banks --
+// groups of 4 lanes -- that take the partner's value); 2 and 1 fall inside a
+// bank and take a quad_perm DPP move plus a select.
+template <int d>
+__device__ __forceinline__ void butterfly(uint32_t& a, uint32_t& b, bool bit) {
+    // After: a(l) = bit ? b(l ^ d) : a(l);  b(l) = bit ? b(l) : a(l ^ d).
+    if constexpr (d == 32) {
+        auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        a = r[0];
+        b = r[1];
+    } else if constexpr (d == 16) {
+        auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        a = r[0];
+        b = r[1];
+    } else if constexpr (d == 8) {
+        // row_ror:8 reads lane l ^ 8 of the 16-lane row; banks 2-3 (lanes
+        // 8-15 of a row) have bit 3 set, banks 0-1 clear.
+        const uint32_t na = (uint32_t)__builtin_amdgcn_update_dpp((int)a, (int)b, 0x128, 0xF, 0xC, false);
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)b, (int)a, 0x128, 0xF, 0x3, false);
+        a = na;
+        b = nb;
+    } else if constexpr (d == 4) {
+        // Banks 1 and 3 (bit 2 set) read lane l - 4 (row_ror:4), banks 0 and
+        // 2 lane l + 4 (row_ror:12); neither wraps around its row.
+        const uint32_t na = (uint32_t)__builtin_amdgcn_update_dpp((int)a, (int)b, 0x124, 0xF, 0xA, false);
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)b, (int)a, 0x12C, 0xF, 0x5, false);
+        a = na;
+        b = nb;
+    } else {
+        static_assert(d == 1 || d == 2, "lane distances are powers of two below 64");
+        constexpr int quad = d == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]: lane l ^ d
+        const uint32_t b_from = (uint32_t)__builtin_amdgcn_mov_dpp((int)b, quad, 0xF, 0xF, false);
+        const uint32_t a_from = (uint32_t)__builtin_amdgcn_mov_dpp((int)a, quad, 0xF, 0xF, false);
+        a = bit ? b_from : a;
+        b = bit ? b : a_from;
+    }
+}
+
+template <int n, int W, int d = 1>
+__device__ __forceinline__ void square_transpose(uint32_t (&v)[n * W], int lane) {
+    if constexpr (d < n) {
+        const bool bit = (lane & d) != 0;
+#pragma unroll
+        for (int b = 0; b < n; ++b) {
+            if (b & d) continue;
+#pragma unroll
+            for (int x = 0; x < W; ++x) butterfly<d>(v[b * W + x], v[(b + d) * W + x], bit);
+        }
+        square_transpose<n, W, 2 * d>(v, lane);
+    }
+}
+
 template <int n, int m>
 __device__ __forceinline__ void partner_table(int id, int (&partner)[m]) {
 #pragma unroll
@@ -76,18 +136,28 @@ __device__ __forceinline__ void partner_table(int id, int (&partner)[m]) {
 }
 
 // Self-contained forms: derive the per-lane tables from the tile rank.
+// Square tiles take the butterfly transpose; every other shape the
+// partner schedule.
 template <int n, int m, int W>
 __device__ __forceinline__ void r2c_words(uint32_t (&v)[m * W], int id, int first) {
-    int partner[m];
-    partner_table<n, m>(id, partner);
-    r2c_words<n, m, W>(v, id, first, partner, tx_schedule<n, m>::unrotate_lane(id));
+    if constexpr (m == n) {
+        square_transpose<n, W>(v, first + id);
+    } else {
+        int partner[m];
+        partner_table<n, m>(id, partner);
+        r2c_words<n, m, W>(v, id, first, partner, tx_schedule<n, m>::unrotate_lane(id));
+    }
 }
 
 template <int n, int m, int W>
 __device__ __forceinline__ void c2r_words(uint32_t (&v)[m * W], int id, int first) {
-    int partner[m];
-    partner_table<n, m>(id, partner);
-    c2r_words<n, m, W>(v, id, first, partner, id % m);
+    if constexpr (m == n) {
+        square_transpose<n, W>(v, first + id);
+    } else {
+        int partner[m];
+        partner_table<n, m>(id, partner);
+        c2r_words<n, m, W>(v, id, first, partner, id % m);
+    }
 }
 
 template <typename T, int i>

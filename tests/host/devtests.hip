@@ -57,6 +57,25 @@ __global__ __launch_bounds__(64) void k_transpose(uint32_t* r2c_out, uint32_t* c
     for (int j = 0; j < M; ++j) c2r_out[lane * M + j] = decode(out[j]);
 }
 
+// The self-contained forms r2c_warp_transpose(v, tile) / c2r_warp_transpose(v,
+// tile): square tiles (m == n) take the butterfly transpose there.
+template <int N, int M, typename T>
+__global__ __launch_bounds__(64) void k_transpose_self(uint32_t* r2c_out, uint32_t* c2r_out) {
+    trove::thread_tile<N> tile;
+    const int r = tile.id();
+    const int lane = threadIdx.x;
+    T in[M];
+    for (int j = 0; j < M; ++j) in[j] = encode<T>((uint32_t)(r + j * N));
+    trove::array<T, M> v = trove::make_array<T, M>(in);
+    trove::r2c_warp_transpose(v, tile);
+    T out[M];
+    trove::make_carray<T, M>(v, out);
+    for (int j = 0; j < M; ++j) r2c_out[lane * M + j] = decode(out[j]);
+    trove::c2r_warp_transpose(v, tile);
+    trove::make_carray<T, M>(v, out);
+    for (int j = 0; j < M; ++j) c2r_out[lane * M + j] = decode(out[j]);
+}
+
 template <int M>
 __global__ __launch_bounds__(64) void k_rotate(uint32_t* out) {
     const int lane = threadIdx.x;
@@ -135,6 +154,20 @@ int devtest_transpose(int n, int m, int word_bytes, uint32_t* r2c_out, uint32_t*
     TT(64, 24, w16) TT(64, 32, w16) TT(32, 32, w16) TT(64, 48, w16) TT(64, 64, w16)
 #undef TT_M
 #undef TT
+    return -1;
+}
+
+// Same cases and contract as devtest_transpose, through the self-contained forms.
+int devtest_transpose_self(int n, int m, int word_bytes, uint32_t* r2c_out, uint32_t* c2r_out) {
+#define TS(N, M, T)                                                                \
+    if (n == N && m == M && word_bytes == (int)sizeof(T)) {                        \
+        hipLaunchKernelGGL((k_transpose_self<N, M, T>), dim3(1), dim3(64), 0, 0, r2c_out, c2r_out); \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                      \
+    }
+    TS(64, 64, uint32_t) TS(64, 64, w16) TS(32, 32, uint32_t) TS(32, 32, w16) TS(16, 16, uint32_t)
+    TS(16, 16, uint64_t) TS(8, 8, uint32_t) TS(4, 4, uint32_t) TS(2, 2, uint32_t) TS(2, 2, w16)
+    TS(64, 5, uint32_t) TS(64, 6, w16) TS(32, 12, uint32_t) TS(64, 48, w16)
+#undef TS
     return -1;
 }
 

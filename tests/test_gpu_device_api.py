@@ -29,6 +29,8 @@ def dt(cuda):
     L = ctypes.CDLL(LIB)
     vp, i, u64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
     L.devtest_transpose.argtypes = [i, i, i, vp, vp]
+    L.devtest_transpose_self.argtypes = [i, i, i, vp, vp]
+    L.devtest_transpose_self.restype = i
     L.devtest_rotate.argtypes = [i, vp]
     L.devtest_converged.argtypes = [u64, vp]
     L.devtest_shfl.argtypes = [i, vp, vp]
@@ -52,6 +54,26 @@ def test_public_transpose_api(dt, n, m, wb):
     r2c = torch.zeros(64 * m, dtype=torch.int32, device="cuda")
     c2r = torch.zeros(64 * m, dtype=torch.int32, device="cuda")
     assert dt.devtest_transpose(n, m, wb, r2c.data_ptr(), c2r.data_ptr()) == 0
+    rank = np.arange(64) % n
+    want_row = (rank[:, None] * m + np.arange(m)[None, :]).astype(np.int64)
+    want_col = (rank[:, None] + np.arange(m)[None, :] * n).astype(np.int64)
+    got_r2c = r2c.cpu().numpy().view(np.uint32).reshape(64, m).astype(np.int64)
+    got_c2r = c2r.cpu().numpy().view(np.uint32).reshape(64, m).astype(np.int64)
+    assert np.array_equal(got_r2c, want_row), (n, m, wb)
+    assert np.array_equal(got_c2r, want_col), (n, m, wb)
+
+
+SELF_CASES = [(64, 64, 4), (64, 64, 16), (32, 32, 4), (32, 32, 16), (16, 16, 4), (16, 16, 8), (8, 8, 4), (4, 4, 4),
+              (2, 2, 4), (2, 2, 16), (64, 5, 4), (64, 6, 16), (32, 12, 4), (64, 48, 16)]
+
+
+@pytest.mark.parametrize("n,m,wb", SELF_CASES)
+def test_self_contained_transpose(dt, n, m, wb):
+    # Square tiles run the butterfly transpose (v_permlane32/16_swap + DPP),
+    # the others the partner schedule; both must meet the same post-condition.
+    r2c = torch.zeros(64 * m, dtype=torch.int32, device="cuda")
+    c2r = torch.zeros(64 * m, dtype=torch.int32, device="cuda")
+    assert dt.devtest_transpose_self(n, m, wb, r2c.data_ptr(), c2r.data_ptr()) == 0
     rank = np.arange(64) % n
     want_row = (rank[:, None] * m + np.arange(m)[None, :]).astype(np.int64)
     want_col = (rank[:, None] + np.arange(m)[None, :] * n).astype(np.int64)
