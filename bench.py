@@ -468,6 +468,8 @@ def main():
                     lclock.run(f"copy_naive_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="direct"))
                     lclock.run(f"gather_{sb}", (2 * sb + 4) * nl,
                                lambda: T.gather(src, idx_l, dst, nl, sb, variant="tile"))
+                    lclock.run(f"gather_coalesced_ptr_{sb}", (2 * sb + 4) * nl,
+                               lambda: T.gather(src, idx_l, dst, nl, sb, variant="shfl"))
                     lclock.run(f"gather_naive_{sb}", (2 * sb + 4) * nl,
                                lambda: T.gather(src, idx_l, dst, nl, sb, variant="direct"))
             torch.cuda.synchronize()
