@@ -111,17 +111,18 @@ int trove_block_write(void* dst, int64_t n_arrays, int s, void* stream);
  * the practical bandwidth ceiling the AoS paths are compared against. */
 int trove_memcpy_ceiling(const void* src, void* dst, int64_t bytes, void* stream);
 
-/* ---- Batched gather: many independent jobs in one persistent launch ----
+/* ---- Batched jobs: many independent gathers / copies in one persistent launch ----
  * No reference counterpart (the reference launches one kernel per
- * benchmark case, tests/benchmark.cu:143-145); each job is the gather of
- * trove_gather. */
+ * benchmark case, tests/benchmark.cu:143-145); a job with indices is the
+ * gather of trove_gather, a job without (idx == NULL) the copy of
+ * trove_copy. */
 typedef struct trove_job {
     const void* src;      /* device: source structs */
-    const int32_t* idx;   /* device: n indices into src */
+    const int32_t* idx;   /* device: n indices into src, or NULL for a copy */
     void* dst;            /* device: n output structs */
-    int64_t n;            /* structs to gather */
+    int64_t n;            /* structs to gather / copy */
     int32_t struct_bytes; /* 4..64, a multiple of 4 */
-    int32_t variant;      /* TROVE_VARIANT_TILE / SHFL / DIRECT (AUTO = TILE) */
+    int32_t variant;      /* gather: TILE / SHFL / DIRECT (AUTO = TILE); copy: DIRECT or else SHFL */
     int32_t chunk_groups; /* 64-struct groups per work item (0 = 16) */
     int32_t reserved;     /* must be 0 */
 } trove_job;
@@ -136,7 +137,8 @@ typedef struct trove_batch trove_batch; /* opaque plan */
 int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out);
 /* Enqueues the whole batch on `stream` in the given mode. */
 int trove_batch_run(trove_batch* b, int mode, void* stream);
-/* Work items and algorithmic bytes ((2 * struct_bytes + 4) * n summed). */
+/* Work items and algorithmic bytes ((2 * struct_bytes + 4) * n per gather,
+ * 2 * struct_bytes * n per copy, summed). */
 int trove_batch_stats(const trove_batch* b, int64_t* work_items, int64_t* algorithmic_bytes);
 int trove_batch_destroy(trove_batch* b);
 

@@ -1,7 +1,8 @@
 """GPU parity of the batched persistent launch and of the gather server.
 
 Batch: every job's output in the persistent, graph-replayed and looped modes
-equals the oracle's gather of that job (bit-exact). Graph replays are checked
+equals the oracle's gather (or, for copy jobs without indices, copy) of that
+job (bit-exact). Graph replays are checked
 one by one with eager work and synchronisation between them.
 Serve: each request's rows equal the oracle's gather, both when the request
 runs alone and when it runs in a mix with others; admission limits (budget,
@@ -27,6 +28,10 @@ def _jobs(count, seed, max_log2=14):
     for j in range(count):
         s, n = sizes[j], int(lengths[j])
         src = rng.integers(0, 256, size=n * s, dtype=np.uint8)
+        if j % 5 == 4:  # a copy job: no indices
+            jobs.append(dict(s=s, n=n, src=src, idx=None, want=oracle.copy(src, n, s),
+                             variant=("shfl", "direct", "tile")[j % 3], chunk=(1, 4, 16, 0)[j % 4]))
+            continue
         idx = T.gen_indices(n, n, patterns[j % 3], seed * 1000 + j)
         jobs.append(dict(s=s, n=n, src=src, idx=idx, want=oracle.gather(src, idx, s),
                          variant=variants[(j // 3) % 3], chunk=(1, 4, 16, 0)[j % 4]))
@@ -36,7 +41,7 @@ def _jobs(count, seed, max_log2=14):
 def _device_jobs(jobs):
     for J in jobs:
         J["d_src"] = torch.from_numpy(J["src"]).cuda()
-        J["d_idx"] = torch.from_numpy(J["idx"]).cuda()
+        J["d_idx"] = None if J["idx"] is None else torch.from_numpy(J["idx"]).cuda()
         J["d_dst"] = torch.zeros(J["n"] * J["s"], dtype=torch.uint8, device="cuda")
     return T.Batch([(J["d_src"], J["d_idx"], J["d_dst"], J["n"], J["s"], J["variant"], J["chunk"]) for J in jobs])
 
@@ -52,7 +57,8 @@ def test_batch_modes(trove, cuda, mode):
     jobs = _jobs(96, 7)
     b = _device_jobs(jobs)
     items, nbytes = b.stats()
-    assert nbytes == sum((2 * J["s"] + 4) * J["n"] for J in jobs) and items > 0
+    assert nbytes == sum((2 * J["s"] + (0 if J["idx"] is None else 4)) * J["n"] for J in jobs) and items > 0
+    assert sum(J["idx"] is None for J in jobs) >= 19
     b.run(mode)
     _check(jobs, mode)
     b.close()

@@ -273,10 +273,11 @@ BATCH_MODES = {"persistent": BATCH_PERSISTENT, "graph": BATCH_GRAPH, "looped": B
 
 
 class Batch:
-    """A validated set of gather jobs runnable as one persistent launch."""
+    """A validated set of gather and copy jobs runnable as one persistent launch."""
 
     def __init__(self, jobs):
-        """jobs: iterable of (src, idx, dst, n, struct_bytes, variant, chunk_groups)."""
+        """jobs: iterable of (src, idx, dst, n, struct_bytes, variant, chunk_groups);
+        idx None makes the job a contiguous copy."""
         arr = (Job * len(jobs))()
         for j, (src, idx, dst, n, b, variant, chunk) in enumerate(jobs):
             arr[j] = Job(_addr(src), _addr(idx), _addr(dst), n, b, _variant(variant), chunk, 0)

@@ -5,9 +5,9 @@
 
 namespace trove_amd {
 
-// One gather job as the kernel sees it. Work item t of the launch belongs to
-// the job with first_item <= t < next job's first_item and covers
-// chunk_groups 64-struct groups of it.
+// One gather job (a copy job when idx is null) as the kernel sees it. Work
+// item t of the launch belongs to the job with first_item <= t < next job's
+// first_item and covers chunk_groups 64-struct groups of it.
 struct job_desc {
     const void* src;
     const int32_t* idx;

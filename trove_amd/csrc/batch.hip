@@ -1,13 +1,15 @@
-// Batched gather: many independent jobs dst_j[i] = src_j[idx_j[i]] of
-// different struct sizes and lengths in ONE persistent launch.
+// Batched jobs: many independent gathers dst_j[i] = src_j[idx_j[i]] and
+// copies dst_j[i] = src_j[i] (idx_j == nullptr) of different struct sizes and
+// lengths in ONE persistent launch.
 //
 // Work is cut into items of `chunk_groups` 64-struct groups of one job; item
 // t belongs to the job j with first_item[j] <= t < first_item[j + 1]. Every
 // wavefront walks the item space with a grid stride, finds its job by a
 // binary search over the (wave-uniform, scalar-loaded) job table and runs the
 // job's variant for its struct size. The launch can be captured once into a
-// hipGraph and replayed. The looped mode launches the standalone gather
-// kernel once per job on the same stream, for comparison at equal output.
+// hipGraph and replayed. The looped mode launches the standalone gather or
+// copy kernel once per job on the same stream, for comparison at equal
+// output.
 #include "batch.h"
 #include "gather_body.h"
 
@@ -20,6 +22,25 @@ namespace trove_amd {
 #define TROVE_BATCH_SIZES(X) \
     X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(36) X(40) X(44) X(48) X(52) X(56) X(60) X(64)
 
+// Copy job items: whole 64-struct groups through load/store_warp_contiguous
+// (SHFL) or per lane (DIRECT); the partial last group per lane, so that the
+// persistent kernel carries no tile cascade.
+template <typename T>
+__device__ __forceinline__ void copy_range(const T* __restrict__ src, T* __restrict__ dst, int64_t n, int64_t g0,
+                                           int64_t g1, bool shfl) {
+    const int lane = trove::lane_id();
+    const int64_t full = n >> 6;
+    for (int64_t g = g0; g < g1; ++g) {
+        const int64_t i = (g << 6) + lane;
+        if (g < full && shfl) {
+            const T v = trove::load_warp_contiguous<64, trove::streaming>(src + i);
+            trove::store_warp_contiguous<64, trove::streaming>(v, dst + i);
+        } else if (i < n) {
+            trove::detail::divergent_store(trove::detail::divergent_load(src + i), dst + i);
+        }
+    }
+}
+
 template <int B>
 __device__ __forceinline__ void batch_item(const job_desc& J, int64_t local) {
     typedef blob_t<B> T;
@@ -28,6 +49,10 @@ __device__ __forceinline__ void batch_item(const job_desc& J, int64_t local) {
     const int64_t groups = (J.n + 63) >> 6;
     const int64_t g0 = local * J.chunk_groups;
     const int64_t g1 = g0 + J.chunk_groups < groups ? g0 + J.chunk_groups : groups;
+    if (J.idx == nullptr) {
+        copy_range<T>(src, dst, J.n, g0, g1, J.variant != TROVE_VARIANT_DIRECT);
+        return;
+    }
     if constexpr (trove::detail::use_shfl<T, 64>::value) {
         if (J.variant == TROVE_VARIANT_TILE) {
             gather_range_tile<T, gather_groups_in_flight<T>()>(src, J.idx, dst, J.n, g0, g1);
@@ -114,7 +139,7 @@ int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out)
             delete b;
             return TROVE_ERR_SIZE;
         }
-        if (J.n < 0 || (J.n > 0 && (!J.src || !J.idx || !J.dst))) {
+        if (J.n < 0 || (J.n > 0 && (!J.src || !J.dst))) {
             delete b;
             return TROVE_ERR_INVALID;
         }
@@ -133,15 +158,21 @@ int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out)
         d.dst = J.dst;
         d.n = J.n;
         d.bytes = bytes;
-        d.variant = J.variant == TROVE_VARIANT_AUTO ? TROVE_VARIANT_TILE : J.variant;
+        const bool copy = J.idx == nullptr;
+        if (copy) {
+            // A copy job: SHFL (load/store_warp_contiguous) unless DIRECT.
+            d.variant = J.variant == TROVE_VARIANT_DIRECT ? TROVE_VARIANT_DIRECT : TROVE_VARIANT_SHFL;
+        } else {
+            d.variant = J.variant == TROVE_VARIANT_AUTO ? TROVE_VARIANT_TILE : J.variant;
+        }
         d.chunk_groups = J.chunk_groups > 0 ? J.chunk_groups : 16;
-        // coalesced_ptr items are one group: a loop around the tile cascade
-        // would keep every level's tables live.
-        if (d.variant == TROVE_VARIANT_SHFL) d.chunk_groups = 1;
+        // coalesced_ptr gather items are one group: a loop around the tile
+        // cascade would keep every level's tables live.
+        if (!copy && d.variant == TROVE_VARIANT_SHFL) d.chunk_groups = 1;
         d.first_item = item;
         const int64_t groups = (J.n + 63) / 64;
         item += (groups + d.chunk_groups - 1) / d.chunk_groups;
-        b->algorithmic_bytes += (2 * (int64_t)bytes + 4) * J.n;
+        b->algorithmic_bytes += (2 * (int64_t)bytes + (copy ? 0 : 4)) * J.n;
         b->table.push_back(d);
     }
     b->total_items = item;
@@ -182,7 +213,8 @@ int trove_batch_run(trove_batch* b, int mode, void* stream) {
     if (mode == TROVE_BATCH_LOOPED) {
         for (const job_desc& d : b->table) {
             if (d.n == 0) continue;
-            const int rc = trove_gather(d.src, d.idx, d.dst, d.n, d.bytes, d.variant, stream);
+            const int rc = d.idx ? trove_gather(d.src, d.idx, d.dst, d.n, d.bytes, d.variant, stream)
+                                 : trove_copy(d.src, d.dst, d.n, d.bytes, d.variant, 64, stream);
             if (rc != TROVE_OK) return rc;
         }
         return TROVE_OK;
