@@ -87,15 +87,35 @@ __device__ __forceinline__ void tile_store_words(T* tile_base, int id,
     }
 }
 
+// A pointer received from another lane, expressed as the caller's own
+// pointer plus the byte distance to it. The address is the same; the
+// pointer keeps `own`'s provenance, so when the compiler knows `own` to be
+// global memory (a kernel argument) the broadcast pointer's accesses are
+// global_load / global_store rather than flat accesses, whose completion
+// every later LDS wait (lgkmcnt) would also have to cover.
+template <typename P>
+__device__ __forceinline__ P* rebase(P* own, uintptr_t target) {
+    typedef typename std::conditional<std::is_const<P>::value, const char, char>::type byte;
+    return reinterpret_cast<P*>(reinterpret_cast<byte*>(own) + (target - reinterpret_cast<uintptr_t>(own)));
+}
+
 // Address of the tile's rank-0 lane, seen by every lane of the tile.
 template <int n, typename P>
 __device__ __forceinline__ P* tile_first_pointer(P* p, int first) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
     if constexpr (n == 64) {
-        return broadcast_first(p);
+        return rebase(p, broadcast_first(a));
     } else {
-        return bpermute(p, first << 2);
+        return rebase(p, bpermute(a, first << 2));
     }
 }
+
+// The dynamic path moves a tile's words in chunks: first every word's owner
+// offset is broadcast (ds_bpermute), then every word's access is issued, so
+// that the broadcasts' latencies overlap instead of each gating one access.
+// A chunk keeps at most kWordChunk broadcasts in flight (the LDS counter
+// tracks 15 outstanding operations).
+constexpr int kWordChunk = 8;
 
 // Word j of rank id in a gathered tile: struct s = (id + j*n) / m, word u.
 // Walks (s, u) incrementally over j.
@@ -116,57 +136,105 @@ struct word_walk {
 // Loads the words of a tile whose structs sit at arbitrary per-lane
 // addresses, in the column-side layout (word j of rank id = tile word
 // id + j*n of the concatenated structs).
+// Word j of rank id belongs to the struct of rank owner[j] (tile-relative)
+// at word word[j]: the word_walk sequence as arrays.
+template <int n, int m>
+__device__ __forceinline__ void word_owners(int id, int (&owner)[m], int (&word)[m]) {
+    word_walk<n, m> w(id);
+#pragma unroll
+    for (int j = 0; j < m; ++j) {
+        owner[j] = w.s;
+        word[j] = w.u;
+        w.next();
+    }
+}
+
+// Word access j goes to base + off[j] (in structs), word word[j]; off[j] is
+// the 32-bit value `key` of lane first + owner[j]. Broadcasts first, then
+// accesses, chunk by chunk.
+template <int n, typename T, bool kStore, typename V>
+__device__ __forceinline__ void tile_words_by_key(T* base, int key, int id, int first, V* v) {
+    typedef word_layout<T> L;
+    typedef typename std::conditional<std::is_const<T>::value, const typename L::U, typename L::U>::type U;
+    int owner[L::m], word[L::m];
+    word_owners<n, L::m>(id, owner, word);
+#pragma unroll
+    for (int j0 = 0; j0 < L::m; j0 += kWordChunk) {
+        constexpr int c = L::m < kWordChunk ? L::m : kWordChunk;
+        int off[c];
+#pragma unroll
+        for (int j = 0; j < c; ++j)
+            if (j0 + j < L::m) off[j] = __builtin_amdgcn_ds_bpermute((first + owner[j0 + j]) << 2, key);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < c; ++j) {
+            if (j0 + j >= L::m) continue;
+            U* p = reinterpret_cast<U*>(base + off[j]) + word[j0 + j];
+            if constexpr (kStore) {
+                typename L::U u;
+                __builtin_memcpy(&u, &v[(j0 + j) * L::W], sizeof(u));
+                *p = u;
+            } else {
+                const typename L::U u = *p;
+                __builtin_memcpy(&v[(j0 + j) * L::W], &u, sizeof(u));
+            }
+        }
+    }
+}
+
+// Same with full per-lane pointers (owner offsets beyond 32 bits): each word
+// receives its owner's pointer.
+template <int n, typename T, bool kStore, typename V>
+__device__ __forceinline__ void tile_words_by_pointer(T* own, int id, int first, V* v) {
+    typedef word_layout<T> L;
+    typedef typename std::conditional<std::is_const<T>::value, const typename L::U, typename L::U>::type U;
+    int owner[L::m], word[L::m];
+    word_owners<n, L::m>(id, owner, word);
+    const uintptr_t a = reinterpret_cast<uintptr_t>(own);
+#pragma unroll
+    for (int j0 = 0; j0 < L::m; j0 += kWordChunk / 2) {
+        constexpr int c = L::m < kWordChunk / 2 ? L::m : kWordChunk / 2;
+        uintptr_t q[c];
+#pragma unroll
+        for (int j = 0; j < c; ++j)
+            if (j0 + j < L::m) q[j] = bpermute(a, (first + owner[j0 + j]) << 2);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < c; ++j) {
+            if (j0 + j >= L::m) continue;
+            U* p = reinterpret_cast<U*>(rebase(own, q[j])) + word[j0 + j];
+            if constexpr (kStore) {
+                typename L::U u;
+                __builtin_memcpy(&u, &v[(j0 + j) * L::W], sizeof(u));
+                *p = u;
+            } else {
+                const typename L::U u = *p;
+                __builtin_memcpy(&v[(j0 + j) * L::W], &u, sizeof(u));
+            }
+        }
+    }
+}
+
+// Loads the words of a tile whose structs sit at arbitrary per-lane
+// addresses, in the column-side layout (word j of rank id = tile word
+// id + j*n of the concatenated structs).
 template <int n, typename T>
 __device__ __forceinline__ void tile_gather_words(const T* src, const T* p0, long long delta, int id, int first,
                                                   uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
-    typedef word_layout<T> L;
-    typedef typename L::U U;
-    word_walk<n, L::m> w(id);
     if (tile_all<n>(delta == (long long)(int)delta)) {
-        const int d32 = (int)delta;
-#pragma unroll
-        for (int j = 0; j < L::m; ++j) {
-            const int ds = __builtin_amdgcn_ds_bpermute((first + w.s) << 2, d32);
-            const U u = reinterpret_cast<const U*>(p0 + ds)[w.u];
-            __builtin_memcpy(&v[j * L::W], &u, sizeof(u));
-            w.next();
-        }
+        tile_words_by_key<n, const T, false>(p0, (int)delta, id, first, v);
     } else {
-#pragma unroll
-        for (int j = 0; j < L::m; ++j) {
-            const T* ps = bpermute(src, (first + w.s) << 2);
-            const U u = reinterpret_cast<const U*>(ps)[w.u];
-            __builtin_memcpy(&v[j * L::W], &u, sizeof(u));
-            w.next();
-        }
+        tile_words_by_pointer<n, const T, false>(src, id, first, v);
     }
 }
 
 template <int n, typename T>
 __device__ __forceinline__ void tile_scatter_words(T* dst, T* p0, long long delta, int id, int first,
                                                    const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
-    typedef word_layout<T> L;
-    typedef typename L::U U;
-    word_walk<n, L::m> w(id);
     if (tile_all<n>(delta == (long long)(int)delta)) {
-        const int d32 = (int)delta;
-#pragma unroll
-        for (int j = 0; j < L::m; ++j) {
-            const int ds = __builtin_amdgcn_ds_bpermute((first + w.s) << 2, d32);
-            U u;
-            __builtin_memcpy(&u, &v[j * L::W], sizeof(u));
-            reinterpret_cast<U*>(p0 + ds)[w.u] = u;
-            w.next();
-        }
+        tile_words_by_key<n, T, true>(p0, (int)delta, id, first, v);
     } else {
-#pragma unroll
-        for (int j = 0; j < L::m; ++j) {
-            T* ps = bpermute(dst, (first + w.s) << 2);
-            U u;
-            __builtin_memcpy(&u, &v[j * L::W], sizeof(u));
-            reinterpret_cast<U*>(ps)[w.u] = u;
-            w.next();
-        }
+        tile_words_by_pointer<n, T, true>(dst, id, first, v);
     }
 }
 
@@ -176,16 +244,7 @@ __device__ __forceinline__ void tile_scatter_words(T* dst, T* p0, long long delt
 template <int n, typename T>
 __device__ __forceinline__ void tile_gather_indexed(const T* base, int index, int id, int first,
                                                     uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
-    typedef word_layout<T> L;
-    typedef typename L::U U;
-    word_walk<n, L::m> w(id);
-#pragma unroll
-    for (int j = 0; j < L::m; ++j) {
-        const int ds = __builtin_amdgcn_ds_bpermute((first + w.s) << 2, index);
-        const U u = reinterpret_cast<const U*>(base + ds)[w.u];
-        __builtin_memcpy(&v[j * L::W], &u, sizeof(u));
-        w.next();
-    }
+    tile_words_by_key<n, const T, false>(base, index, id, first, v);
 }
 
 // Tile scatter into one array: the structs held in column-side layout go to
@@ -193,17 +252,7 @@ __device__ __forceinline__ void tile_gather_indexed(const T* base, int index, in
 template <int n, typename T>
 __device__ __forceinline__ void tile_scatter_indexed(T* base, int index, int id, int first,
                                                      const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
-    typedef word_layout<T> L;
-    typedef typename L::U U;
-    word_walk<n, L::m> w(id);
-#pragma unroll
-    for (int j = 0; j < L::m; ++j) {
-        const int ds = __builtin_amdgcn_ds_bpermute((first + w.s) << 2, index);
-        U u;
-        __builtin_memcpy(&u, &v[j * L::W], sizeof(u));
-        reinterpret_cast<U*>(base + ds)[w.u] = u;
-        w.next();
-    }
+    tile_words_by_key<n, T, true>(base, index, id, first, v);
 }
 
 }  // namespace detail
