@@ -41,9 +41,25 @@ __device__ __forceinline__ void copy_range(const T* __restrict__ src, T* __restr
     }
 }
 
+// Job pointers come from the job table in memory, so the compiler cannot
+// tell their address space and would use flat accesses (whose completion
+// every LDS wait of the transposition and the index broadcast also covers).
+// The C ABI takes device memory only, so each is rebuilt from its address as
+// a global-memory pointer (an integer-to-pointer conversion into address
+// space 1; a pointer-to-pointer round trip would be folded away).
+template <typename P>
+__device__ __forceinline__ P* assume_global(P* p) {
+    typedef __attribute__((address_space(1))) P global_p;
+    return (P*)(global_p*)reinterpret_cast<uintptr_t>(p);
+}
+
 template <int B>
-__device__ __forceinline__ void batch_item(const job_desc& J, int64_t local) {
+__device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local) {
     typedef blob_t<B> T;
+    job_desc J = J0;
+    J.src = assume_global(J.src);
+    J.dst = assume_global(J.dst);
+    if (J.idx) J.idx = assume_global(J.idx);
     const T* src = static_cast<const T*>(J.src);
     T* dst = static_cast<T*>(J.dst);
     const int64_t groups = (J.n + 63) >> 6;
