@@ -25,4 +25,16 @@ static_assert(sizeof(job_desc) == 56, "job_desc layout");
 // first_item, total_items work items in all).
 int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, hipStream_t s);
 
+// Up to kArgJobs jobs travel in the kernel arguments instead (56 * 64 + 16
+// bytes, inside the 4 KiB argument segment): no table upload on the launch
+// path.
+constexpr int kArgJobs = 64;
+struct job_pack {
+    job_desc jobs[kArgJobs];
+    int32_t n_jobs;
+    int32_t pad;
+    int64_t total_items;
+};
+int launch_job_pack(const job_pack& p, hipStream_t s);
+
 }  // namespace trove_amd

@@ -82,13 +82,11 @@ __device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local) {
     gather_range_direct<T>(src, J.idx, dst, J.n, g0, g1);
 }
 
-// One work item per wavefront, no loop over items: in a loop, LICM hoists
-// the per-lane tables of every (size, variant) branch and keeps them all
-// live (275 VGPRs). The grid has exactly total_items wavefronts.
-__global__ __launch_bounds__(kBlock) void k_batch_gather(const job_desc* __restrict__ jobs, int n_jobs,
-                                                         int64_t total_items) {
-    const int64_t it = wave_index();
-    if (it >= total_items) return;
+// Work item `it` of a job table (device memory or kernel arguments): its job
+// by binary search on first_item (wave-uniform, scalar loads), then the
+// job's path for its struct size.
+template <typename Table>
+__device__ __forceinline__ void run_item(const Table& jobs, int n_jobs, int64_t it) {
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -112,10 +110,30 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather(const job_desc* __restr
     }
 }
 
+// One work item per wavefront, no loop over items: in a loop, LICM hoists
+// the per-lane tables of every (size, variant) branch and keeps them all
+// live (275 VGPRs). The grid has exactly total_items wavefronts.
+__global__ __launch_bounds__(kBlock) void k_batch_gather(const job_desc* __restrict__ jobs, int n_jobs,
+                                                         int64_t total_items) {
+    const int64_t it = wave_index();
+    if (it < total_items) run_item(jobs, n_jobs, it);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_gather_pack(const job_pack p) {
+    const int64_t it = wave_index();
+    if (it < p.total_items) run_item(p.jobs, p.n_jobs, it);
+}
+
 int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, hipStream_t s) {
     // One wavefront per item: never capped.
     const int64_t blocks = (total_items + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL(k_batch_gather, dim3((unsigned)blocks), dim3(kBlock), 0, s, d_table, n_jobs, total_items);
+    return launch_status();
+}
+
+int launch_job_pack(const job_pack& p, hipStream_t s) {
+    const int64_t blocks = (p.total_items + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL(k_batch_gather_pack, dim3((unsigned)blocks), dim3(kBlock), 0, s, p);
     return launch_status();
 }
 

@@ -13,6 +13,8 @@
 //    persistent batch-gather kernel, so requests admitted while others are
 //    still on the GPU join the next step and their work overlaps on the
 //    device;
+//  * a step of up to 64 requests passes its job table in the kernel
+//    arguments; larger steps upload it from pinned memory first;
 //  * pipelining: up to kInflight steps are queued on the compute stream;
 //    host index uploads run on a separate copy stream and the compute stream
 //    waits on them by event;
@@ -125,6 +127,7 @@ struct trove_serve {
     job_desc* h_jobs[kInflight] = {};  // pinned
     job_desc* d_jobs[kInflight] = {};
     int max_jobs = 0;
+    job_pack pack = {};  // kernel-argument job table of small steps
     uint64_t step_seq = 0;  // next step slot = step_seq % kInflight
     uint64_t retired_seq = 0;
     std::vector<uint64_t> newly_done;  // completions not yet seen by replay
@@ -239,9 +242,19 @@ static int launch_step(trove_serve* s) {
     }
     if (nj == 0) return TROVE_OK;
     s->active.swap(still_active);
-    int rc = hip_status(
-        hipMemcpyAsync(s->d_jobs[k], jobs, sizeof(job_desc) * nj, hipMemcpyHostToDevice, s->stream));
-    if (rc == TROVE_OK) rc = launch_job_table(s->d_jobs[k], nj, items, s->stream);
+    int rc = TROVE_OK;
+    if (nj <= kArgJobs) {
+        // Small steps carry their jobs in the kernel arguments: no upload.
+        job_pack& p = s->pack;
+        for (int j = 0; j < nj; ++j) p.jobs[j] = jobs[j];
+        p.n_jobs = nj;
+        p.pad = 0;
+        p.total_items = items;
+        rc = launch_job_pack(p, s->stream);
+    } else {
+        rc = hip_status(hipMemcpyAsync(s->d_jobs[k], jobs, sizeof(job_desc) * nj, hipMemcpyHostToDevice, s->stream));
+        if (rc == TROVE_OK) rc = launch_job_table(s->d_jobs[k], nj, items, s->stream);
+    }
     if (rc == TROVE_OK) rc = hip_status(hipEventRecord(st.done, s->stream));
     if (rc != TROVE_OK) return rc;
     st.busy = true;
