@@ -23,7 +23,9 @@
 //      at word (l + j*tile) % m; its address comes from lane s. When every
 //      lane's offset from p0 fits in 32 bits (the usual case: one array) the
 //      offset is broadcast with one ds_bpermute instead of a 64-bit pointer
-//      (two).
+//      (two). The broadcasts of a chunk of words are issued before their
+//      accesses, and broadcast pointers keep the caller's pointer provenance
+//      (global memory accesses when the compiler knows the array is global).
 #pragma once
 #include <trove/array.h>
 #include <trove/memory.h>
