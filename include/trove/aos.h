@@ -175,9 +175,9 @@ __device__ __forceinline__ void tile_words_by_key(T* base, int key, int id, int 
             if constexpr (kStore) {
                 typename L::U u;
                 __builtin_memcpy(&u, &v[(j0 + j) * L::W], sizeof(u));
-                *p = u;
+                store_word<cached>(p, u);
             } else {
-                const typename L::U u = *p;
+                const typename L::U u = load_word<cached>(p);
                 __builtin_memcpy(&v[(j0 + j) * L::W], &u, sizeof(u));
             }
         }
@@ -208,9 +208,9 @@ __device__ __forceinline__ void tile_words_by_pointer(T* own, int id, int first,
             if constexpr (kStore) {
                 typename L::U u;
                 __builtin_memcpy(&u, &v[(j0 + j) * L::W], sizeof(u));
-                *p = u;
+                store_word<cached>(p, u);
             } else {
-                const typename L::U u = *p;
+                const typename L::U u = load_word<cached>(p);
                 __builtin_memcpy(&v[(j0 + j) * L::W], &u, sizeof(u));
             }
         }

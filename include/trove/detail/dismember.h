@@ -49,19 +49,25 @@ struct streaming {};
 
 namespace detail {
 
+// One word access. Words of 8 and 16 bytes move as one 2- / 4-dword vector
+// (aligned to its size, like word2 / word4), never as separate fields: a
+// struct-typed access is split into field accesses early on, and when the
+// compiler then merges the code tails of two paths, the fields get separate
+// addresses and are no longer re-joined into one dwordx4 access.
 template <typename Policy, typename U>
 __device__ __forceinline__ U load_word(const U* p) {
-    if constexpr (std::is_same<Policy, streaming>::value && sizeof(U) % 4 == 0) {
+    if constexpr (sizeof(U) % 4 == 0) {
         constexpr int W = sizeof(U) / 4;
-        U u;
-        if constexpr (W == 1) {
-            const uint32_t t = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
-            __builtin_memcpy(&u, &t, sizeof(U));
+        typedef typename std::conditional<W == 1, uint32_t, uint32_t __attribute__((ext_vector_type(W)))>::type vec;
+        const vec* q = reinterpret_cast<const vec*>(p);
+        vec t;
+        if constexpr (std::is_same<Policy, streaming>::value) {
+            t = __builtin_nontemporal_load(q);
         } else {
-            typedef uint32_t vec __attribute__((ext_vector_type(W)));
-            const vec t = __builtin_nontemporal_load(reinterpret_cast<const vec*>(p));
-            __builtin_memcpy(&u, &t, sizeof(U));
+            t = *q;
         }
+        U u;
+        __builtin_memcpy(&u, &t, sizeof(U));
         return u;
     } else {
         return *p;
@@ -70,17 +76,15 @@ __device__ __forceinline__ U load_word(const U* p) {
 
 template <typename Policy, typename U>
 __device__ __forceinline__ void store_word(U* p, const U& u) {
-    if constexpr (std::is_same<Policy, streaming>::value && sizeof(U) % 4 == 0) {
+    if constexpr (sizeof(U) % 4 == 0) {
         constexpr int W = sizeof(U) / 4;
-        if constexpr (W == 1) {
-            uint32_t t;
-            __builtin_memcpy(&t, &u, sizeof(U));
-            __builtin_nontemporal_store(t, reinterpret_cast<uint32_t*>(p));
-        } else {
-            typedef uint32_t vec __attribute__((ext_vector_type(W)));
-            vec t;
-            __builtin_memcpy(&t, &u, sizeof(U));
+        typedef typename std::conditional<W == 1, uint32_t, uint32_t __attribute__((ext_vector_type(W)))>::type vec;
+        vec t;
+        __builtin_memcpy(&t, &u, sizeof(U));
+        if constexpr (std::is_same<Policy, streaming>::value) {
             __builtin_nontemporal_store(t, reinterpret_cast<vec*>(p));
+        } else {
+            *reinterpret_cast<vec*>(p) = t;
         }
     } else {
         *p = u;
