@@ -124,8 +124,8 @@ constexpr int kWordChunk = 8;
 template <int n, int m>
 struct word_walk {
     int s, u;
-    __device__ __forceinline__ explicit word_walk(int id) : s(id / m), u(id % m) {}
-    __device__ __forceinline__ void next() {
+    __host__ __device__ __forceinline__ explicit word_walk(int id) : s(id / m), u(id % m) {}
+    __host__ __device__ __forceinline__ void next() {
         u += n % m;
         if (u >= m) {
             u -= m;
@@ -141,7 +141,7 @@ struct word_walk {
 // Word j of rank id belongs to the struct of rank owner[j] (tile-relative)
 // at word word[j]: the word_walk sequence as arrays.
 template <int n, int m>
-__device__ __forceinline__ void word_owners(int id, int (&owner)[m], int (&word)[m]) {
+__host__ __device__ __forceinline__ void word_owners(int id, int (&owner)[m], int (&word)[m]) {
     word_walk<n, m> w(id);
 #pragma unroll
     for (int j = 0; j < m; ++j) {

@@ -5,8 +5,11 @@
 // sequence of trove/transpose.h, driven by tx_schedule<n, m>, must map the
 // column-side layout (lane a, slot b = a + b*n) to the row-side layout
 // (lane c, slot d = c*m + d), and the C2R sequence must invert it, with the
-// C2R pushes landing on distinct (lane, slot) pairs. Exits non-zero on the
-// first failure.
+// C2R pushes landing on distinct (lane, slot) pairs. The dynamic path's word
+// walk (trove/aos.h: word j of rank id belongs to struct (id + j*n) / m at
+// word (id + j*n) % m) must equal that integer division for every rank.
+// Exits non-zero on the first failure.
+#include <trove/aos.h>
 #include <trove/array.h>
 #include <trove/detail/schedule.h>
 #include <trove/rotate.h>
@@ -88,6 +91,25 @@ static int all_m() {
     }
 }
 
+template <int n, int m>
+static int walk_checks() {
+    if constexpr (m > n) {
+        return 0;
+    } else {
+        int fails = 0;
+        for (int id = 0; id < n; ++id) {
+            int owner[m], word[m];
+            trove::detail::word_owners<n, m>(id, owner, word);
+            for (int j = 0; j < m; ++j)
+                if (owner[j] != (id + j * n) / m || word[j] != (id + j * n) % m) {
+                    if (!fails) std::printf("word walk n=%d m=%d id %d j %d\n", n, m, id, j);
+                    ++fails;
+                }
+        }
+        return fails + walk_checks<n, m + 1>();
+    }
+}
+
 template <int m>
 static int rotate_checks() {
     int fails = 0;
@@ -109,6 +131,8 @@ int main() {
     int f = all_m<2, 1>() + all_m<4, 1>() + all_m<8, 1>() + all_m<16, 1>() + all_m<32, 1>() + all_m<64, 1>();
     f += rotate_checks<2>() + rotate_checks<3>() + rotate_checks<5>() + rotate_checks<7>() + rotate_checks<12>() +
          rotate_checks<15>() + rotate_checks<16>();
-    std::printf("schedule/rotate failures: %d\n", f);
+    f += walk_checks<2, 1>() + walk_checks<4, 1>() + walk_checks<8, 1>() + walk_checks<16, 1>() +
+         walk_checks<32, 1>() + walk_checks<64, 1>();
+    std::printf("schedule/rotate/word-walk failures: %d\n", f);
     return f == 0 ? 0 : 1;
 }
