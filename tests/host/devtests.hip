@@ -128,6 +128,24 @@ __global__ __launch_bounds__(64) void k_cascade(uint64_t mask, int scatter, cons
     }
 }
 
+// coalesced_ptr over a __shared__ array: active lanes (mask) read struct
+// perm[lane] of 64 structs staged in LDS, through the dynamic path (pointers
+// into LDS are flat pointers to the LDS aperture).
+template <int S>
+__global__ __launch_bounds__(64) void k_shared_gather(uint64_t mask, const trove::array<uint32_t, S / 4>* src,
+                                                      const int* perm, trove::array<uint32_t, S / 4>* dst) {
+    typedef trove::array<uint32_t, S / 4> T;
+    __shared__ uint32_t words[64 * (S / 4)];  // raw storage: array<> has a constructor
+    T* staged = reinterpret_cast<T*>(words);
+    const int lane = threadIdx.x;
+    staged[lane] = src[lane];
+    __syncthreads();
+    if ((mask >> lane) & 1ull) {
+        trove::coalesced_ptr<T> p(staged);
+        dst[lane] = p[perm[lane]];
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -207,6 +225,18 @@ int devtest_cascade(int s, uint64_t mask, int scatter, const void* src, const in
     }
     CS(8) CS(12) CS(20) CS(32) CS(48) CS(60) CS(96) CS(384) CS(512) CS(1024)
 #undef CS
+    return -1;
+}
+
+int devtest_shared_gather(int s, uint64_t mask, const void* src, const int* perm, void* dst) {
+#define SG(S)                                                                                            \
+    if (s == S) {                                                                                        \
+        hipLaunchKernelGGL((k_shared_gather<S>), dim3(1), dim3(64), 0, 0, mask,                          \
+                           (const trove::array<uint32_t, S / 4>*)src, perm, (trove::array<uint32_t, S / 4>*)dst); \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                                            \
+    }
+    SG(8) SG(20) SG(32) SG(60) SG(64)
+#undef SG
     return -1;
 }
 

@@ -35,6 +35,8 @@ def dt(cuda):
     L.devtest_converged.argtypes = [u64, vp]
     L.devtest_shfl.argtypes = [i, vp, vp]
     L.devtest_cascade.argtypes = [i, u64, i, vp, vp, vp]
+    L.devtest_shared_gather.argtypes = [i, u64, vp, vp, vp]
+    L.devtest_shared_gather.restype = i
     for f in (L.devtest_transpose, L.devtest_rotate, L.devtest_converged, L.devtest_shfl, L.devtest_cascade):
         f.restype = i
     return L
@@ -158,3 +160,23 @@ def test_cascade_under_masks(dt, s):
                         else:
                             want[lane] = src2[perm[lane]]
                 assert np.array_equal(got, want), (s, hex(mask), scatter)
+
+
+@pytest.mark.parametrize("s", [8, 20, 32, 60, 64])
+def test_coalesced_ptr_over_lds(dt, s):
+    # The dynamic path on pointers into a __shared__ array (LDS aperture).
+    rng = np.random.default_rng(100 + s)
+    k = s // 4
+    for mask in _masks()[:12]:
+        perm = rng.permutation(64).astype(np.int32)
+        src = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
+        init = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
+        d_dst = torch.from_numpy(init.view(np.int32)).cuda()
+        assert dt.devtest_shared_gather(s, mask, torch.from_numpy(src.view(np.int32)).cuda().data_ptr(),
+                                        torch.from_numpy(perm).cuda().data_ptr(), d_dst.data_ptr()) == 0
+        got = d_dst.cpu().numpy().view(np.uint32).reshape(64, k)
+        want = init.reshape(64, k).copy()
+        for lane in range(64):
+            if (mask >> lane) & 1:
+                want[lane] = src.reshape(64, k)[perm[lane]]
+        assert np.array_equal(got, want), (s, hex(mask))
