@@ -6,6 +6,10 @@ namespace trove_amd {
 // Lanes of one 64-array group that a seeded hash sends down the fallback
 // path before the convergence test: one group in eight is affected, and in
 // it each lane with probability 1/2.
+__device__ __forceinline__ bool divergent_group(uint64_t seed, int64_t group) {
+    return (splitmix64(seed ^ (uint64_t)group) & 7) == 0;
+}
+
 __device__ __forceinline__ bool divergent_lane(uint64_t seed, int64_t index) {
     const uint64_t h = splitmix64(seed ^ (uint64_t)(index >> 6));
     if ((h & 7) != 0) return false;
@@ -28,19 +32,36 @@ __global__ __launch_bounds__(64) void k_block_copy(const uint32_t* __restrict__ 
     typedef trove::array<uint32_t, S> s_ary;
     constexpr int U = block_shape<S>::unroll;
     const int64_t unit = S >= 5 ? unit_for_xcd(blockIdx.x, (int64_t)gridDim.x) : (int64_t)blockIdx.x;
+    // Every group of the unit whole, undisturbed and converged: all loads
+    // before all stores, so that U loads are in flight per wavefront.
+    bool whole = U > 1 && ((unit * U + U) << 6) <= l;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int64_t index = ((unit * U + u) << 6) + threadIdx.x;
-        if (index < l) {
-            if (divergent && divergent_lane(seed, index)) {
-                const s_ary d = trove::load_array<S>(x, index);
-                trove::store_array(r, index, d);
-            } else if (trove::warp_converged()) {
-                const s_ary d = trove::load_array_warp_contiguous<S, trove::WARP_SIZE, trove::streaming>(x, index);
-                trove::store_array_warp_contiguous<S, trove::WARP_SIZE, trove::streaming>(r, index, d);
-            } else {
-                const s_ary d = trove::load_array<S>(x, index);
-                trove::store_array(r, index, d);
+    for (int u = 0; u < U; ++u) whole = whole && !(divergent && divergent_group(seed, unit * U + u));
+    if (whole && trove::warp_converged()) {
+        s_ary d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            d[u] = trove::load_array_warp_contiguous<S, trove::WARP_SIZE, trove::streaming>(
+                x, ((unit * U + u) << 6) + threadIdx.x);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            trove::store_array_warp_contiguous<S, trove::WARP_SIZE, trove::streaming>(
+                r, ((unit * U + u) << 6) + threadIdx.x, d[u]);
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t index = ((unit * U + u) << 6) + threadIdx.x;
+            if (index < l) {
+                if (divergent && divergent_lane(seed, index)) {
+                    const s_ary d = trove::load_array<S>(x, index);
+                    trove::store_array(r, index, d);
+                } else if (trove::warp_converged()) {
+                    const s_ary d = trove::load_array_warp_contiguous<S, trove::WARP_SIZE, trove::streaming>(x, index);
+                    trove::store_array_warp_contiguous<S, trove::WARP_SIZE, trove::streaming>(r, index, d);
+                } else {
+                    const s_ary d = trove::load_array<S>(x, index);
+                    trove::store_array(r, index, d);
+                }
             }
         }
     }
