@@ -263,3 +263,21 @@ def test_full_size_large_structs(trove, cuda, s):
         assert _equal_in_chunks(back, d_src), variant
     del d_src, d_dst, d_idx, back
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("s", [1, 2, 5, 15])
+def test_full_size_block_interface(trove, cuda, s):
+    # The block-interface workload at its full shape: 2^28 32-bit elements,
+    # s per lane (2^28 % s trailing elements), converged and with one group in
+    # eight diverged; the copy must be the identity.
+    e = 1 << 28
+    d_x = torch.empty(e, dtype=torch.int32, device=cuda)
+    T.fill_words_device(d_x, e, 31 + s)
+    d_r = torch.empty_like(d_x)
+    for divergent in (False, True):
+        d_r.fill_(-1)
+        T.block_copy(d_x, d_r, e, s, divergent=divergent, seed=77 + s)
+        torch.cuda.synchronize()
+        assert _equal_in_chunks(d_r, d_x), (s, divergent)
+    del d_x, d_r
+    torch.cuda.empty_cache()
