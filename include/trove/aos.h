@@ -217,24 +217,42 @@ __device__ __forceinline__ void tile_words_by_pointer(T* own, int id, int first,
     }
 }
 
+// The lane's address relative to its tile's rank-0 address, in structs.
+// `exact` is false when the byte distance is not a whole number of structs
+// (lanes pointing into different arrays); such a tile never takes the
+// struct-offset shortcuts. Integer arithmetic on purpose: a C++ pointer
+// difference is an exact division, which lets the compiler assume the
+// distance is always whole.
+struct tile_offset {
+    long long delta;
+    bool exact;
+};
+
+template <typename T>
+__device__ __forceinline__ tile_offset offset_from(const T* p, const T* p0) {
+    const long long bytes = (long long)(reinterpret_cast<uintptr_t>(p) - reinterpret_cast<uintptr_t>(p0));
+    const long long delta = bytes / (long long)sizeof(T);
+    return {delta, delta * (long long)sizeof(T) == bytes};
+}
+
 // Loads the words of a tile whose structs sit at arbitrary per-lane
 // addresses, in the column-side layout (word j of rank id = tile word
 // id + j*n of the concatenated structs).
 template <int n, typename T>
-__device__ __forceinline__ void tile_gather_words(const T* src, const T* p0, long long delta, int id, int first,
+__device__ __forceinline__ void tile_gather_words(const T* src, const T* p0, tile_offset o, int id, int first,
                                                   uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
-    if (tile_all<n>(delta == (long long)(int)delta)) {
-        tile_words_by_key<n, const T, false>(p0, (int)delta, id, first, v);
+    if (tile_all<n>(o.exact && o.delta == (long long)(int)o.delta)) {
+        tile_words_by_key<n, const T, false>(p0, (int)o.delta, id, first, v);
     } else {
         tile_words_by_pointer<n, const T, false>(src, id, first, v);
     }
 }
 
 template <int n, typename T>
-__device__ __forceinline__ void tile_scatter_words(T* dst, T* p0, long long delta, int id, int first,
+__device__ __forceinline__ void tile_scatter_words(T* dst, T* p0, tile_offset o, int id, int first,
                                                    const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
-    if (tile_all<n>(delta == (long long)(int)delta)) {
-        tile_words_by_key<n, T, true>(p0, (int)delta, id, first, v);
+    if (tile_all<n>(o.exact && o.delta == (long long)(int)o.delta)) {
+        tile_words_by_key<n, T, true>(p0, (int)o.delta, id, first, v);
     } else {
         tile_words_by_pointer<n, T, true>(dst, id, first, v);
     }
@@ -329,12 +347,12 @@ __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value, 
     const int id = tile.id();
     const int first = tile.first_lane();
     const T* p0 = tile_first_pointer<n>(src, first);
-    const long long delta = (long long)(src - p0);
+    const tile_offset o = offset_from(src, p0);
     uint32_t v[L::m * L::W];
-    if (tile_all<n>(delta == id)) {
+    if (tile_all<n>(o.exact && o.delta == id)) {
         tile_load_words<n>(p0, id, v);
     } else {
-        tile_gather_words<n>(src, p0, delta, id, first, v);
+        tile_gather_words<n>(src, p0, o, id, first, v);
     }
     r2c_words<n, L::m, L::W>(v, id, first);
     T out;
@@ -355,14 +373,14 @@ __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value>:
     const int id = tile.id();
     const int first = tile.first_lane();
     T* p0 = tile_first_pointer<n>(dest, first);
-    const long long delta = (long long)(dest - p0);
+    const tile_offset o = offset_from(dest, p0);
     uint32_t v[L::m * L::W];
     struct_to_words(data, v);
     c2r_words<n, L::m, L::W>(v, id, first);
-    if (tile_all<n>(delta == id)) {
+    if (tile_all<n>(o.exact && o.delta == id)) {
         tile_store_words<n>(p0, id, v);
     } else {
-        tile_scatter_words<n>(dest, p0, delta, id, first, v);
+        tile_scatter_words<n>(dest, p0, o, id, first, v);
     }
 }
 

@@ -146,6 +146,21 @@ __global__ __launch_bounds__(64) void k_shared_gather(uint64_t mask, const trove
     }
 }
 
+// Per-lane pointers into two views of one buffer that start `shift` bytes
+// apart (a word, not a whole struct): lane l loads the struct at byte
+// base + shift_l + perm[l] * S through trove::load and stores it to
+// dst[l] through trove::store (the dynamic path on both sides).
+template <int S>
+__global__ __launch_bounds__(64) void k_mixed(uint64_t mask, const unsigned char* base, const int* perm,
+                                              const int* shift, trove::array<uint32_t, S / 4>* dst) {
+    typedef trove::array<uint32_t, S / 4> T;
+    const int lane = threadIdx.x;
+    if ((mask >> lane) & 1ull) {
+        const T* p = reinterpret_cast<const T*>(base + shift[lane] + (long)perm[lane] * S);
+        trove::store(trove::load(p), dst + lane);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -237,6 +252,18 @@ int devtest_shared_gather(int s, uint64_t mask, const void* src, const int* perm
     }
     SG(8) SG(20) SG(32) SG(60) SG(64)
 #undef SG
+    return -1;
+}
+
+int devtest_mixed(int s, uint64_t mask, const void* base, const int* perm, const int* shift, void* dst) {
+#define MX(S)                                                                                              \
+    if (s == S) {                                                                                          \
+        hipLaunchKernelGGL((k_mixed<S>), dim3(1), dim3(64), 0, 0, mask, (const unsigned char*)base, perm, shift, \
+                           (trove::array<uint32_t, S / 4>*)dst);                                           \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                                              \
+    }
+    MX(12) MX(20) MX(32) MX(48) MX(96)
+#undef MX
     return -1;
 }
 

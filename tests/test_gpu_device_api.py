@@ -37,6 +37,8 @@ def dt(cuda):
     L.devtest_cascade.argtypes = [i, u64, i, vp, vp, vp]
     L.devtest_shared_gather.argtypes = [i, u64, vp, vp, vp]
     L.devtest_shared_gather.restype = i
+    L.devtest_mixed.argtypes = [i, u64, vp, vp, vp, vp]
+    L.devtest_mixed.restype = i
     for f in (L.devtest_transpose, L.devtest_rotate, L.devtest_converged, L.devtest_shfl, L.devtest_cascade):
         f.restype = i
     return L
@@ -180,3 +182,35 @@ def test_coalesced_ptr_over_lds(dt, s):
             if (mask >> lane) & 1:
                 want[lane] = src.reshape(64, k)[perm[lane]]
         assert np.array_equal(got, want), (s, hex(mask))
+
+
+@pytest.mark.parametrize("s", [12, 20, 32, 48, 96])
+def test_dynamic_path_pointers_a_word_apart(dt, s):
+    # Lanes of one tile point into two views of a buffer that start one word
+    # (not a whole struct) apart: the struct-offset shortcuts must not apply.
+    word = 16 if s % 16 == 0 else 8 if s % 8 == 0 else 4
+    rng = np.random.default_rng(300 + s)
+    k = s // 4
+    buf = rng.integers(0, 256, size=64 * s + 64, dtype=np.uint8)
+    d_buf = torch.from_numpy(buf).cuda()
+    for mask in _masks()[:10]:
+        for pattern in ("alternate", "halves", "random"):
+            perm = rng.permutation(64).astype(np.int32)
+            if pattern == "alternate":
+                shift = (np.arange(64) % 2) * word
+            elif pattern == "halves":
+                shift = (np.arange(64) >= 32) * word
+            else:
+                shift = rng.integers(0, 2, size=64) * word
+            shift = shift.astype(np.int32)
+            init = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
+            d_dst = torch.from_numpy(init.view(np.int32)).cuda()
+            assert dt.devtest_mixed(s, mask, d_buf.data_ptr(), torch.from_numpy(perm).cuda().data_ptr(),
+                                    torch.from_numpy(shift).cuda().data_ptr(), d_dst.data_ptr()) == 0
+            got = d_dst.cpu().numpy().view(np.uint32).reshape(64, k)
+            want = init.reshape(64, k).copy()
+            for lane in range(64):
+                if (mask >> lane) & 1:
+                    off = int(shift[lane]) + int(perm[lane]) * s
+                    want[lane] = buf[off:off + s].view(np.uint32)
+            assert np.array_equal(got, want), (s, hex(mask), pattern)
