@@ -418,6 +418,15 @@ def main():
                            lambda: T.scatter(src, idx, dst, n, s, variant="direct"))
         for _ in range(3):
             nclock.run("memcpy_ceiling", 2 * n * smax, lambda: T.memcpy_ceiling(src, dst, n * smax))
+        # Random whole 128-byte lines of the 4 GiB source: the rate the
+        # random gathers' line fills are bounded by.
+        n_lines = n * smax // 128
+        n_reads = 1 << 26
+        line_idx = torch.from_numpy(T.gen_indices(n_reads, n_lines, "random", args.seed + 41)).to(dev)
+        line_out = torch.empty(n_reads, dtype=torch.int32, device=dev)
+        for _ in range(3):
+            nclock.run("random_lines_ceiling", 128 * n_reads,
+                       lambda: T.random_lines_ceiling(src, n_lines, line_idx, line_out, n_reads))
         # Block interface: 2^28 32-bit elements, s per lane, converged and
         # with one wavefront group in eight diverged.
         e = 1 << 28
@@ -581,6 +590,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "memcpy_ceiling_gbs": naive.get("memcpy_ceiling", {}).get("gbs"),
+        "random_line_ceiling_gbs": naive.get("random_lines_ceiling", {}).get("gbs"),
         "block": block,
         "large_structs": large,
         "sweep": sweep,

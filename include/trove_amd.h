@@ -111,6 +111,14 @@ int trove_block_write(void* dst, int64_t n_arrays, int s, void* stream);
  * the practical bandwidth ceiling the AoS paths are compared against. */
 int trove_memcpy_ceiling(const void* src, void* dst, int64_t bytes, void* stream);
 
+/* Random 128-byte line reads: out[r] = XOR of the 32 words of line
+ * min(line_idx[r], n_lines - 1) of buf (128-byte aligned, n_lines lines), with
+ * 64 lines in flight per wavefront: the rate at which the device
+ * serves random line fills, the ceiling of random gathers. No reference
+ * counterpart (benchmark plumbing like trove_memcpy_ceiling). */
+int trove_random_lines_ceiling(const void* buf, int64_t n_lines, const uint32_t* line_idx, uint32_t* out,
+                               int64_t n_reads, void* stream);
+
 /* ---- Batched jobs: many independent gathers / copies in one persistent launch ----
  * No reference counterpart (the reference launches one kernel per
  * benchmark case, tests/benchmark.cu:143-145); a job with indices is the

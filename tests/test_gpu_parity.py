@@ -179,6 +179,21 @@ def test_memcpy_ceiling_and_fill(trove, cuda):
     assert torch.equal(r, d)
 
 
+@pytest.mark.parametrize("n_reads", [1, 255, 256, 257, 10007])
+def test_random_lines_ceiling_values(trove, cuda, n_reads):
+    # out[r] = XOR of the 32 words of line min(idx[r], n_lines - 1) (indices
+    # beyond the buffer read its last line instead of outside it).
+    n_lines = 4096
+    words = T.gen_words(n_lines * 32, 5)
+    rng = np.random.default_rng(n_reads)
+    idx = rng.integers(0, 3 * n_lines, size=n_reads, dtype=np.int64).astype(np.uint32)
+    d_buf = dev(words.view(np.int32))
+    out = torch.zeros(n_reads, dtype=torch.int32, device=cuda)
+    T.random_lines_ceiling(d_buf, n_lines, dev(idx.view(np.int32)), out, n_reads)
+    want = np.bitwise_xor.reduce(words.reshape(n_lines, 32)[np.minimum(idx, n_lines - 1)], axis=1)
+    assert np.array_equal(host(out).view(np.uint32), want)
+
+
 def test_small_contiguous_config_bit_exact(trove, cuda):
     # Contiguous AoS copy, 20-byte structs, N = 2^20, against the scalar loop.
     n, s = 1 << 20, 20

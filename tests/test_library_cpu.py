@@ -85,6 +85,11 @@ def test_validation_before_launch():
     assert L.trove_block_copy(fake, fake + 4, 10, 4, 0, 0, None) == T.ERR_ALIGN
     assert L.trove_block_write(fake, 10, 0, None) == T.ERR_INVALID
     assert L.trove_memcpy_ceiling(fake, fake, 24, None) == T.ERR_INVALID
+    assert L.trove_random_lines_ceiling(fake, 0, fake, fake, 10, None) == T.ERR_INVALID
+    assert L.trove_random_lines_ceiling(fake, 1 << 32, fake, fake, 10, None) == T.ERR_INVALID
+    assert L.trove_random_lines_ceiling(fake + 64, 8, fake, fake, 10, None) == T.ERR_ALIGN
+    assert L.trove_random_lines_ceiling(fake, 8, fake, fake, -1, None) == T.ERR_INVALID
+    assert L.trove_random_lines_ceiling(0, 8, 0, 0, 0, None) == T.OK
     with pytest.raises(T.TroveError):
         T.copy(fake, fake, 10, 68, stream=0)
 

@@ -78,6 +78,7 @@ _SIGNATURES = {
     "trove_block_copy": ([_vp, _vp, _i64, _int, _int, _u64, _vp], _int),
     "trove_block_write": ([_vp, _i64, _int, _vp], _int),
     "trove_memcpy_ceiling": ([_vp, _vp, _i64, _vp], _int),
+    "trove_random_lines_ceiling": ([_vp, _i64, _vp, _vp, _i64, _vp], _int),
     "trove_gen_words": ([_vp, _i64, _u64], _int),
     "trove_gen_permutation": ([_vp, _i64, _u64], _int),
     "trove_fill_words_device": ([_vp, _i64, _u64, _vp], _int),
@@ -201,6 +202,12 @@ def block_write(dst, n_arrays: int, s: int, stream=None) -> None:
 
 def memcpy_ceiling(src, dst, nbytes: int, stream=None) -> None:
     _check(lib().trove_memcpy_ceiling(_addr(src), _addr(dst), nbytes, _stream(stream)), "trove_memcpy_ceiling")
+
+
+def random_lines_ceiling(buf, n_lines: int, line_idx, out, n_reads: int, stream=None) -> None:
+    """out[r] = XOR of the 32 words of 128-byte line min(line_idx[r], n_lines - 1) of buf."""
+    _check(lib().trove_random_lines_ceiling(_addr(buf), n_lines, _addr(line_idx), _addr(out), n_reads,
+                                            _stream(stream)), "trove_random_lines_ceiling")
 
 
 def fill_words_device(out, n: int, seed: int, stream=None) -> None:

@@ -12,6 +12,8 @@ int block_copy_dispatch(const uint32_t* x, uint32_t* r, int64_t e, int s, int di
                         hipStream_t st);
 int block_write_dispatch(uint32_t* r, int64_t l, int s, hipStream_t st);
 int memcpy_ceiling_launch(const void* src, void* dst, int64_t bytes, hipStream_t s);
+int random_lines_launch(const void* buf, int64_t n_lines, const uint32_t* line_idx, uint32_t* out, int64_t n,
+                        hipStream_t s);
 int fill_words_launch(uint32_t* out, int64_t n, uint64_t seed, hipStream_t s);
 uint32_t host_word_hash(uint64_t seed, uint64_t i);
 
@@ -225,6 +227,15 @@ int trove_memcpy_ceiling(const void* src, void* dst, int64_t bytes, void* stream
     if (!src || !dst) return TROVE_ERR_INVALID;
     if (!aligned(src, 16) || !aligned(dst, 16)) return TROVE_ERR_ALIGN;
     return memcpy_ceiling_launch(src, dst, bytes, static_cast<hipStream_t>(stream));
+}
+
+int trove_random_lines_ceiling(const void* buf, int64_t n_lines, const uint32_t* line_idx, uint32_t* out,
+                               int64_t n_reads, void* stream) {
+    if (n_lines <= 0 || n_lines > 0xffffffffLL || n_reads < 0 || n_reads > kMaxStructs) return TROVE_ERR_INVALID;
+    if (n_reads == 0) return TROVE_OK;
+    if (!buf || !line_idx || !out) return TROVE_ERR_INVALID;
+    if (!aligned(buf, 128) || !aligned(line_idx, 4) || !aligned(out, 4)) return TROVE_ERR_ALIGN;
+    return random_lines_launch(buf, n_lines, line_idx, out, n_reads, static_cast<hipStream_t>(stream));
 }
 
 int trove_gen_words(uint32_t* out, int64_t n, uint64_t seed) {
