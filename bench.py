@@ -421,16 +421,21 @@ def main():
         # Random whole 128-byte lines of the 4 GiB source: the rate the
         # random gathers' line fills are bounded by.
         n_lines = n * smax // 128
-        n_reads = 1 << 26
-        line_idx = torch.from_numpy(T.gen_indices(n_reads, n_lines, "random", args.seed + 41)).to(dev)
-        line_out = torch.empty(n_reads, dtype=torch.int32, device=dev)
-        for _ in range(3):
-            nclock.run("random_lines_ceiling", 128 * n_reads,
-                       lambda: T.random_lines_ceiling(src, n_lines, line_idx, line_out, n_reads))
+        if n_lines >= 1024:
+            n_reads = 1 << 26
+            line_idx = torch.from_numpy(T.gen_indices(n_reads, n_lines, "random", args.seed + 41)).to(dev)
+            line_out = torch.empty(n_reads, dtype=torch.int32, device=dev)
+            for _ in range(3):
+                nclock.run("random_lines_ceiling", 128 * n_reads,
+                           lambda: T.random_lines_ceiling(src, n_lines, line_idx, line_out, n_reads))
+            del line_idx, line_out
         # Block interface: 2^28 32-bit elements, s per lane, converged and
         # with one wavefront group in eight diverged.
         e = 1 << 28
-        for sb in (1, 3, 5, 7, 9, 11, 13, 15):
+        block_fits = e <= src.numel() and e <= dst.numel()  # 1 GiB each way; the default sweep has 4 GiB
+        if not block_fits:
+            log(f"bench: block interface leg skipped (buffers sized for --k {args.k})")
+        for sb in (1, 3, 5, 7, 9, 11, 13, 15) if block_fits else ():
             for _ in range(2):
                 nclock.run(f"block_s{sb}", 8 * e, lambda: T.block_copy(src, dst, e, sb))
                 nclock.run(f"block_div_s{sb}", 8 * e, lambda: T.block_copy(src, dst, e, sb, divergent=True, seed=7))
@@ -467,9 +472,11 @@ def main():
     if naive:
         lclock = KernelClock(torch)
         for nl, sizes in ((1 << 24, (96, 128, 160, 192, 224, 256)), (1 << 22, (384, 512, 768, 1024))):
-            # Up to 4 GiB of rows = the size of src / dst; indices are their
-            # own permutation of [0, nl).
-            assert nl * max(sizes) <= src.numel() * 4 and nl * max(sizes) <= dst.numel() * 4
+            # Up to 4 GiB of rows = the size of src / dst at the default sweep
+            # (k up to 16); indices are their own permutation of [0, nl).
+            if nl * max(sizes) > src.numel() * 4 or nl * max(sizes) > dst.numel() * 4:
+                log(f"bench: large structs {sizes[0]}..{sizes[-1]} B skipped (buffers sized for --k {args.k})")
+                continue
             idx_l = torch.from_numpy(T.gen_permutation(nl, args.seed + 31)).to(dev)
             for sb in sizes:
                 for _ in range(3):
@@ -489,7 +496,7 @@ def main():
         for name, (launches, ms, nbytes) in lclock.totals.items():
             large[name + "_gbs"] = round(nbytes / launches / (ms / launches * 1e-3) / 1e9, 1)
     block = None
-    if naive:
+    if naive and "block_s1" in naive:
         block = {"workload": "block interface copy, 2^28 x 32-bit elements, s per lane; divergent: 1 in 8 "
                              "64-array groups with lanes masked (warp_converged fallback)",
                  "gbs": {f"s{sb}": naive[f"block_s{sb}"]["gbs"] for sb in (1, 3, 5, 7, 9, 11, 13, 15)},
