@@ -418,17 +418,6 @@ def main():
                            lambda: T.scatter(src, idx, dst, n, s, variant="direct"))
         for _ in range(3):
             nclock.run("memcpy_ceiling", 2 * n * smax, lambda: T.memcpy_ceiling(src, dst, n * smax))
-        # Random whole 128-byte lines of the 4 GiB source: the rate the
-        # random gathers' line fills are bounded by.
-        n_lines = n * smax // 128
-        if n_lines >= 1024:
-            n_reads = 1 << 26
-            line_idx = torch.from_numpy(T.gen_indices(n_reads, n_lines, "random", args.seed + 41)).to(dev)
-            line_out = torch.empty(n_reads, dtype=torch.int32, device=dev)
-            for _ in range(3):
-                nclock.run("random_lines_ceiling", 128 * n_reads,
-                           lambda: T.random_lines_ceiling(src, n_lines, line_idx, line_out, n_reads))
-            del line_idx, line_out
         # Block interface: 2^28 32-bit elements, s per lane, converged and
         # with one wavefront group in eight diverged.
         e = 1 << 28
@@ -439,6 +428,18 @@ def main():
             for _ in range(2):
                 nclock.run(f"block_s{sb}", 8 * e, lambda: T.block_copy(src, dst, e, sb))
                 nclock.run(f"block_div_s{sb}", 8 * e, lambda: T.block_copy(src, dst, e, sb, divergent=True, seed=7))
+        # Random whole 128-byte lines of the 4 GiB source: the rate the
+        # random gathers' line fills are bounded by (last: it leaves the
+        # source's pages scattered through the caches and TLBs).
+        n_lines = n * smax // 128
+        if n_lines >= 1024:
+            n_reads = 1 << 26
+            line_idx = torch.from_numpy(T.gen_indices(n_reads, n_lines, "random", args.seed + 41)).to(dev)
+            line_out = torch.empty(n_reads, dtype=torch.int32, device=dev)
+            for _ in range(3):
+                nclock.run("random_lines_ceiling", 128 * n_reads,
+                           lambda: T.random_lines_ceiling(src, n_lines, line_idx, line_out, n_reads))
+            del line_idx, line_out
         torch.cuda.synchronize()
         nclock.collect()
         for name, (launches, ms, nbytes) in nclock.totals.items():
