@@ -411,11 +411,15 @@ def main():
                 nclock.run(f"copy_direct_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="direct"))
                 nclock.run(f"gather_direct_k{k}", (2 * s + 4) * n,
                            lambda: T.gather(src, idx, dst, n, s, variant="direct"))
+                # The LDS-staged variant (trove/lds.h) beside the transposition.
+                nclock.run(f"copy_lds_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="lds"))
+                nclock.run(f"gather_lds_k{k}", (2 * s + 4) * n, lambda: T.gather(src, idx, dst, n, s, variant="lds"))
             for _ in range(2):
                 nclock.run(f"scatter_tile_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="tile"))
                 nclock.run(f"scatter_shfl_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="shfl"))
                 nclock.run(f"scatter_direct_k{k}", (2 * s + 4) * n,
                            lambda: T.scatter(src, idx, dst, n, s, variant="direct"))
+                nclock.run(f"scatter_lds_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="lds"))
         for _ in range(3):
             nclock.run("memcpy_ceiling", 2 * n * smax, lambda: T.memcpy_ceiling(src, dst, n * smax))
         # Block interface: 2^28 32-bit elements, s per lane, converged and
@@ -464,6 +468,8 @@ def main():
             row.update({"scatter_tile_gbs": naive[f"scatter_tile_k{k}"]["gbs"],
                         "scatter_coalesced_ptr_gbs": naive[f"scatter_shfl_k{k}"]["gbs"],
                         "scatter_naive_gbs": naive[f"scatter_direct_k{k}"]["gbs"]})
+            row["lds_gbs"] = {"copy": naive[f"copy_lds_k{k}"]["gbs"], "gather": naive[f"gather_lds_k{k}"]["gbs"],
+                              "scatter": naive[f"scatter_lds_k{k}"]["gbs"]}
         row["copy_frac_peak"] = round(row["copy_gbs"] / HBM_PEAK_GBS, 3)
         sweep.append(row)
     # Structs beyond the sweep: 96..1024 bytes (6..64 16-byte words; 96, 160,
@@ -489,6 +495,10 @@ def main():
                                lambda: T.gather(src, idx_l, dst, nl, sb, variant="shfl"))
                     lclock.run(f"gather_naive_{sb}", (2 * sb + 4) * nl,
                                lambda: T.gather(src, idx_l, dst, nl, sb, variant="direct"))
+                    if T.lds_supported(sb):
+                        lclock.run(f"copy_lds_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="lds"))
+                        lclock.run(f"gather_lds_{sb}", (2 * sb + 4) * nl,
+                                   lambda: T.gather(src, idx_l, dst, nl, sb, variant="lds"))
             torch.cuda.synchronize()
             del idx_l
         lclock.collect()

@@ -46,10 +46,12 @@ extern "C" {
 #define TROVE_ERR_RANGE (-11)   /* index out of range (where indices are checked) */
 
 /* Kernel variants. */
-#define TROVE_VARIANT_AUTO 0   /* the library's choice (tuned table) */
+#define TROVE_VARIANT_AUTO 0   /* the library's per-size choice (measured on MI355X; see each entry point) */
 #define TROVE_VARIANT_SHFL 1   /* Trove API path: warp-contiguous transposition / coalesced_ptr */
 #define TROVE_VARIANT_DIRECT 2 /* naive per-lane compiler-generated struct access */
 #define TROVE_VARIANT_TILE 3   /* gather/scatter: tile-indexed word access, transposition elided */
+#define TROVE_VARIANT_LDS 4    /* LDS-staged transposition (trove/lds.h): dwordx4 tile traffic, per-lane LDS
+                                  reads; struct sizes that are multiples of 4 up to 512 bytes */
 
 /* Message for a return code. */
 const char* trove_strerror(int code);
@@ -63,22 +65,29 @@ int trove_struct_size_supported(int struct_bytes);
 int trove_struct_word_bytes(int struct_bytes);
 
 /* dst[i] = src[i], i < n: one struct per lane through load_warp_contiguous /
- * store_warp_contiguous<tile> (variant SHFL; tile in {2,4,...,64}, 0 = 64) or
- * the naive per-lane copy (DIRECT). Replaces the launches of
- * tests/benchmark.cu:74-88 (contiguous load) and :52-72 (contiguous store)
- * combined into the copy the configs measure; device side: aos.h:66-105. */
+ * store_warp_contiguous<tile> (variant SHFL; tile in {2,4,...,64}, 0 = 64),
+ * through load/store_warp_contiguous_lds (LDS; tile 0 or 64) or the naive
+ * per-lane copy (DIRECT). AUTO: LDS for 4, 8, 12, 16, 32, 96, 128 and 160
+ * bytes, else SHFL (SHFL whenever tile is below 64). Replaces the launches
+ * of tests/benchmark.cu:74-88 (contiguous load) and :52-72 (contiguous
+ * store) combined into one copy; device side: aos.h:66-105. */
 int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int variant, int tile, void* stream);
 
 /* dst[i] = src[idx[i]], i < n. SHFL: `T d = coalesced_ptr(src)[idx[i]];
  * coalesced_ptr(dst)[i] = d;` (tests/benchmark.cu:90-99); DIRECT: the naive
  * gather (tests/benchmark.cu:112-119); TILE: tile-indexed loads + coalesced
- * stores with the transposition elided. */
+ * stores with the transposition elided (4-, 8- and 16-byte structs: per-lane
+ * row words); LDS: the tile-indexed loads staged through LDS to one struct
+ * per lane and back out with dwordx4 stores (the LDS-staged counterpart of
+ * SHFL). AUTO: LDS for 16, 48, 64 and 192 bytes, else TILE. */
 int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
                  void* stream);
 
 /* dst[idx[i]] = src[i], i < n, idx distinct. SHFL: tests/benchmark.cu:101-110;
  * DIRECT: tests/benchmark.cu:121-128; TILE: coalesced loads + tile-indexed
- * stores with the transposition elided. */
+ * stores with the transposition elided; LDS: dwordx4 loads staged through
+ * LDS to one struct per lane, then tile-indexed stores. AUTO: DIRECT for 4
+ * bytes, LDS for 8 and 16, else TILE. */
 int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
                   void* stream);
 
@@ -130,7 +139,8 @@ typedef struct trove_job {
     void* dst;            /* device: n output structs */
     int64_t n;            /* structs to gather / copy */
     int32_t struct_bytes; /* 4..64, a multiple of 4 */
-    int32_t variant;      /* gather: TILE / SHFL / DIRECT (AUTO = TILE); copy: DIRECT or else SHFL */
+    int32_t variant;      /* gather: TILE / SHFL / LDS / DIRECT; copy: SHFL (also for TILE) / LDS / DIRECT;
+                             AUTO: the per-size choice of trove_gather / trove_copy */
     int32_t chunk_groups; /* 64-struct groups per work item (0 = 16) */
     int32_t reserved;     /* must be 0 */
 } trove_job;
@@ -162,8 +172,10 @@ typedef struct trove_serve trove_serve; /* opaque server */
 
 /* table: device rows of row_bytes bytes (4..64, multiple of 4), n_rows of
  * them. budget_bytes bounds the output rows of admitted requests (the index
- * arena is budget_bytes / row_bytes * 4 bytes more). Work runs on `stream`;
- * host index uploads run on a private stream. */
+ * arena is budget_bytes / row_bytes * 4 bytes more). variant: the gather
+ * path of every step (TILE, SHFL, LDS, DIRECT, or AUTO: trove_gather's
+ * per-size choice). Work runs on
+ * `stream`; host index uploads run on a private stream. */
 int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int64_t budget_bytes,
                        int32_t max_slots, int64_t slice_indices, int32_t variant, void* stream,
                        trove_serve** out);

@@ -4,6 +4,7 @@
 // the host test computes the expected values.
 #include <hip/hip_runtime.h>
 #include <trove/aos.h>
+#include <trove/lds.h>
 #include <trove/ptr.h>
 
 #include <cstdint>
@@ -161,6 +162,26 @@ __global__ __launch_bounds__(64) void k_mixed(uint64_t mask, const unsigned char
     }
 }
 
+// LDS-staged warp-contiguous API (trove/lds.h) in a work-group of WAVES
+// wavefronts, each with its own stage: wavefront w of block b handles tile
+// t = b * WAVES + w at byte offset `off` of src / dst. Lane l stores the
+// struct load_warp_contiguous_lds gave it to got[t * 64 + l] with a plain
+// per-lane store, then writes it back through store_warp_contiguous_lds.
+template <int S, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_lds_roundtrip(const unsigned char* src, int off,
+                                                              trove::array<uint32_t, S / 4>* got, unsigned char* dst) {
+    typedef trove::array<uint32_t, S / 4> T;
+    __shared__ __align__(16) uint32_t stage[WAVES * 64 * (S / 4)];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long t = (long)blockIdx.x * WAVES + w;
+    const T* s = reinterpret_cast<const T*>(src + off) + t * 64;
+    T* d = reinterpret_cast<T*>(dst + off) + t * 64;
+    uint32_t* my = stage + w * 64 * (S / 4);
+    const T v = trove::load_warp_contiguous_lds(s + lane, my);
+    got[t * 64 + lane] = v;
+    trove::store_warp_contiguous_lds(v, d + lane, my);
+}
+
 }  // namespace
 
 extern "C" {
@@ -252,6 +273,21 @@ int devtest_shared_gather(int s, uint64_t mask, const void* src, const int* perm
     }
     SG(8) SG(20) SG(32) SG(60) SG(64)
 #undef SG
+    return -1;
+}
+
+// tiles must be a multiple of the work-group's wavefront count (4 up to 64
+// bytes, 1 above).
+int devtest_lds(int s, int off, int tiles, const void* src, void* got, void* dst) {
+#define LR(S, W)                                                                                          \
+    if (s == S) {                                                                                         \
+        if (tiles % W) return -1;                                                                         \
+        hipLaunchKernelGGL((k_lds_roundtrip<S, W>), dim3(tiles / W), dim3(64 * W), 0, 0,                  \
+                           (const unsigned char*)src, off, (trove::array<uint32_t, S / 4>*)got, (unsigned char*)dst); \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                                             \
+    }
+    LR(4, 4) LR(8, 4) LR(12, 4) LR(20, 4) LR(32, 4) LR(60, 4) LR(64, 4) LR(96, 1) LR(256, 1)
+#undef LR
     return -1;
 }
 

@@ -23,18 +23,18 @@ def _jobs(count, seed, max_log2=14):
     sizes = [4 * (1 + j % 16) for j in range(count)]
     lengths = T.gen_loguniform(count, 1 << 6, 1 << max_log2, seed)
     patterns = ["identity", "random", "runs"]
-    variants = ["tile", "shfl", "direct"]
+    variants = ["tile", "shfl", "direct", "lds"]
     jobs = []
     for j in range(count):
         s, n = sizes[j], int(lengths[j])
         src = rng.integers(0, 256, size=n * s, dtype=np.uint8)
         if j % 5 == 4:  # a copy job: no indices
             jobs.append(dict(s=s, n=n, src=src, idx=None, want=oracle.copy(src, n, s),
-                             variant=("shfl", "direct", "tile")[j % 3], chunk=(1, 4, 16, 0)[j % 4]))
+                             variant=("shfl", "direct", "tile", "lds")[j % 4], chunk=(1, 4, 16, 0)[(j // 20) % 4]))
             continue
         idx = T.gen_indices(n, n, patterns[j % 3], seed * 1000 + j)
         jobs.append(dict(s=s, n=n, src=src, idx=idx, want=oracle.gather(src, idx, s),
-                         variant=variants[(j // 3) % 3], chunk=(1, 4, 16, 0)[j % 4]))
+                         variant=variants[(j // 3) % 4], chunk=(1, 4, 16, 0)[j % 4]))
     return jobs
 
 
@@ -113,6 +113,21 @@ def test_serve_alone_equals_mix(trove, cuda):
         alone.drain()
         assert np.array_equal(alone.fetch(rid, r.size), mixed[k]), ("alone", k)
         alone.close()
+
+
+def test_serve_variants_agree(trove, cuda):
+    # Every gather path of the server's steps returns the oracle's rows.
+    n_rows, rb = 1 << 14, 32
+    host_table, table = _table(n_rows, rb, 8)
+    sizes = T.gen_loguniform(16, 1 << 4, 1 << 11, 12)
+    reqs = [T.gen_zipf(int(n), n_rows, 1.1, 300 + i) for i, n in enumerate(sizes)]
+    for variant in ("tile", "shfl", "lds", "direct"):
+        srv = T.Server(table, n_rows, rb, budget_bytes=1 << 20, max_slots=4, slice_indices=128, variant=variant)
+        ids = [srv.submit(r, r.size, on_device=False) for r in reqs]
+        srv.drain()
+        for k, (i, r) in enumerate(zip(ids, reqs)):
+            assert np.array_equal(srv.fetch(i, r.size), oracle.gather(host_table, r, rb)), (variant, k)
+        srv.close()
 
 
 def test_serve_device_indices_and_replay(trove, cuda):

@@ -9,7 +9,9 @@
   "every lane of my aligned N-group is active" (warp.h:56-93);
 * thread_tile<N>::shfl reads from the addressed rank of the caller's tile;
 * trove::load / trove::store under arbitrary EXEC masks move exactly the
-  active lanes' structs (the tile cascade, aos.h:258-290).
+  active lanes' structs (the tile cascade, aos.h:258-290);
+* load_warp_contiguous_lds / store_warp_contiguous_lds (trove/lds.h, no
+  reference counterpart) put struct l in lane l and write it back.
 """
 import ctypes
 import os
@@ -39,6 +41,8 @@ def dt(cuda):
     L.devtest_shared_gather.restype = i
     L.devtest_mixed.argtypes = [i, u64, vp, vp, vp, vp]
     L.devtest_mixed.restype = i
+    L.devtest_lds.argtypes = [i, i, i, vp, vp, vp]
+    L.devtest_lds.restype = i
     for f in (L.devtest_transpose, L.devtest_rotate, L.devtest_converged, L.devtest_shfl, L.devtest_cascade):
         f.restype = i
     return L
@@ -182,6 +186,31 @@ def test_coalesced_ptr_over_lds(dt, s):
             if (mask >> lane) & 1:
                 want[lane] = src.reshape(64, k)[perm[lane]]
         assert np.array_equal(got, want), (s, hex(mask))
+
+
+@pytest.mark.parametrize("s", [4, 8, 12, 20, 32, 60, 64, 96, 256])
+def test_lds_staged_api(dt, s):
+    # load_warp_contiguous_lds gives lane l struct l of its tile (checked by a
+    # plain per-lane store) and store_warp_contiguous_lds writes it back, with
+    # several wavefronts per work-group on private stages; tile bases 16-byte
+    # aligned (dwordx4 chunks) and 4 or 8 bytes off (dword chunks). Bytes
+    # around the tiles stay untouched.
+    k = s // 4
+    tiles = 8
+    rng = np.random.default_rng(500 + s)
+    for off in (0, 4, 8):
+        nbytes = off + tiles * 64 * s + 16
+        src = rng.integers(0, 256, size=nbytes, dtype=np.uint8)
+        init = rng.integers(0, 256, size=nbytes, dtype=np.uint8)
+        d_src = torch.from_numpy(src).cuda()
+        d_dst = torch.from_numpy(init.copy()).cuda()
+        d_got = torch.zeros(tiles * 64 * k, dtype=torch.int32, device="cuda")
+        assert dt.devtest_lds(s, off, tiles, d_src.data_ptr(), d_got.data_ptr(), d_dst.data_ptr()) == 0
+        body = src[off:off + tiles * 64 * s]
+        assert np.array_equal(d_got.cpu().numpy().view(np.uint8), body), (s, off, "lane l holds struct l")
+        want = init.copy()
+        want[off:off + tiles * 64 * s] = body
+        assert np.array_equal(d_dst.cpu().numpy(), want), (s, off, "store")
 
 
 @pytest.mark.parametrize("s", [12, 20, 32, 48, 96])

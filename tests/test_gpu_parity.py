@@ -42,11 +42,16 @@ def test_copy_all_variants(trove, cuda, s):
     src = rand_bytes(n * s, s)
     want = oracle.copy(src, n, s)
     d_src = dev(src)
-    for variant, tiles in (("shfl", TILES), ("direct", (64,))):
+    cases = (("shfl", TILES), ("direct", (64,))) + ((("lds", (64,)),) if T.lds_supported(s) else ())
+    for variant, tiles in cases:
         for tile in tiles:
             d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
             T.copy(d_src, d_dst, n, s, variant=variant, tile=tile)
             assert np.array_equal(host(d_dst), want), (s, variant, tile)
+    if not T.lds_supported(s):
+        with pytest.raises(T.TroveError) as e:
+            T.copy(d_src, d_src, n, s, variant="lds")
+        assert e.value.code == T.ERR_VARIANT
 
 
 @pytest.mark.parametrize("s", [4, 12, 20, 32, 60, 64, 96, 160])
@@ -54,9 +59,10 @@ def test_copy_partial_wavefront_cascade(trove, cuda, s):
     # Every trailing-group length 1..64 runs the tile cascade once.
     for n in list(range(1, 66)) + [127, 129]:
         src = rand_bytes(n * s, 1000 + n)
-        d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
-        T.copy(dev(src), d_dst, n, s, variant="shfl")
-        assert np.array_equal(host(d_dst), src), (s, n)
+        for variant in ("shfl", "lds"):
+            d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+            T.copy(dev(src), d_dst, n, s, variant=variant)
+            assert np.array_equal(host(d_dst), src), (s, n, variant)
 
 
 def _patterns(n, n_src, seed):
@@ -82,7 +88,7 @@ def test_gather_all_variants(trove, cuda, s):
     for name, idx in _patterns(n, n_src, s).items():
         want = oracle.gather(src, idx, s)
         d_idx = dev(idx)
-        for variant in ("shfl", "tile", "direct"):
+        for variant in ("shfl", "tile", "direct") + (("lds",) if T.lds_supported(s) else ()):
             d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
             T.gather(d_src, d_idx, d_dst, n, s, variant=variant)
             assert np.array_equal(host(d_dst), want), (s, name, variant)
@@ -97,10 +103,66 @@ def test_scatter_all_variants(trove, cuda, s):
     for name, idx in (("perm", np.random.default_rng(s).permutation(n).astype(np.int32)),
                       ("identity", np.arange(n, dtype=np.int32))):
         want = oracle.scatter(src, idx, init, s)
-        for variant in ("shfl", "tile", "direct"):
+        for variant in ("shfl", "tile", "direct") + (("lds",) if T.lds_supported(s) else ()):
             d_dst = dev(init.copy())
             T.scatter(d_src, dev(idx), d_dst, n, s, variant=variant)
             assert np.array_equal(host(d_dst), want), (s, name, variant)
+
+
+def test_auto_variant_every_size(trove, cuda):
+    # AUTO picks a path per struct size (LDS-staged or not); whatever it picks
+    # must match the oracle, and a sub-wavefront tile with AUTO must work.
+    n, n_src = 64 * 11 + 29, 1500
+    for s in SIZES:
+        src = rand_bytes(n_src * s, 40 + s)
+        d_src = dev(src)
+        for tile in (64, 32):
+            d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+            T.copy(d_src, d_dst, n, s, variant="auto", tile=tile)
+            assert np.array_equal(host(d_dst), src[:n * s]), (s, "copy", tile)
+        idx = np.random.default_rng(s).integers(0, n_src, size=n, dtype=np.int32)
+        d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+        T.gather(d_src, dev(idx), d_dst, n, s, variant="auto")
+        assert np.array_equal(host(d_dst), oracle.gather(src, idx, s)), (s, "gather")
+        perm = np.random.default_rng(s + 1).permutation(n).astype(np.int32)
+        init = rand_bytes(n * s, 41 + s)
+        d_dst = dev(init.copy())
+        T.scatter(d_src, dev(perm), d_dst, n, s, variant="auto")
+        assert np.array_equal(host(d_dst), oracle.scatter(src[:n * s], perm, init, s)), (s, "scatter")
+
+
+@pytest.mark.parametrize("s", [4, 8, 12, 20, 24, 40, 44, 56, 60])
+def test_lds_unaligned_tile_bases(trove, cuda, s):
+    # Tile bases that are word-aligned but not 16-byte aligned take the LDS
+    # variant's dword-chunk path: copy (both sides), gather (destination) and
+    # scatter (source) at byte offsets of one and two dwords, against the
+    # oracle; bytes outside the views stay untouched.
+    w = T.word_bytes(s)
+    n, n_src = 64 * 19 + 21, 2500
+    src = rand_bytes(n_src * s, s + 3)
+    idx = np.random.default_rng(s).integers(0, n_src, size=n, dtype=np.int32)
+    perm = np.random.default_rng(s + 1).permutation(n).astype(np.int32)
+    for off in (4, 8):
+        if off % w:
+            continue
+        pad = rand_bytes(n_src * s + 16, off)
+        d_src = dev(pad.copy())
+        d_src[off:off + n_src * s] = dev(src)
+        d_out = dev(pad.copy())
+        T.copy(d_src[off:], d_out[off:], n, s, variant="lds")
+        want = pad.copy()
+        want[off:off + n * s] = src[:n * s]
+        assert np.array_equal(host(d_out), want), (s, off, "copy")
+        d_out = dev(pad.copy())
+        T.gather(d_src[off:], dev(idx), d_out[off:], n, s, variant="lds")
+        want = pad.copy()
+        want[off:off + n * s] = oracle.gather(src, idx, s)
+        assert np.array_equal(host(d_out), want), (s, off, "gather")
+        d_out = dev(pad.copy())
+        T.scatter(d_src[off:], dev(perm), d_out[off:], n, s, variant="lds")
+        want = pad.copy()
+        want[off:off + n * s] = oracle.scatter(src[:n * s], perm, pad[off:off + n * s], s)
+        assert np.array_equal(host(d_out), want), (s, off, "scatter")
 
 
 def test_index_golden(trove, cuda):
@@ -109,7 +171,7 @@ def test_index_golden(trove, cuda):
     for k in range(1, 17):
         s = 4 * k
         d_src = dev(np.arange(n * k, dtype=np.uint32).view(np.uint8))
-        for variant in ("shfl", "tile", "direct"):
+        for variant in ("shfl", "tile", "direct", "lds"):
             d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
             T.gather(d_src, dev(perm), d_dst, n, s, variant=variant)
             assert np.array_equal(host(d_dst).view(np.uint32), g[f"gather_k{k}"]), (k, variant)
@@ -223,9 +285,11 @@ def test_full_size_sweep_properties(trove, cuda, k):
     d_src = torch.empty(n * k, dtype=torch.int32, device=cuda)
     T.fill_words_device(d_src, n * k, 77 + k)
     d_dst = torch.empty_like(d_src)
-    T.copy(d_src, d_dst, n, s, variant="shfl")
-    torch.cuda.synchronize()
-    assert _equal_in_chunks(d_dst, d_src)
+    for variant in ("shfl", "lds"):
+        d_dst.zero_()
+        T.copy(d_src, d_dst, n, s, variant=variant)
+        torch.cuda.synchronize()
+        assert _equal_in_chunks(d_dst, d_src), variant
     perm = T.gen_permutation(n, 4242)
     d_idx = dev(perm)
     rows = np.random.default_rng(k).integers(0, n, size=4096)
@@ -234,7 +298,7 @@ def test_full_size_sweep_properties(trove, cuda, k):
     del src_host
     src2d = d_src.view(n, k)
     step = (1 << 26) // k
-    for variant in ("tile", "shfl", "direct"):
+    for variant in ("tile", "shfl", "direct", "lds"):
         d_dst.zero_()
         T.gather(d_src, d_idx, d_dst, n, s, variant=variant)
         torch.cuda.synchronize()
@@ -264,14 +328,15 @@ def test_full_size_large_structs(trove, cuda, s):
     d_idx = dev(T.gen_permutation(n, 99 + s))
     src2d = d_src.view(n, k)
     step = max(1, (1 << 26) // k)
-    for variant in ("tile", "shfl"):
+    variants = ("tile", "shfl") + (("lds",) if T.lds_supported(s) else ())
+    for variant in variants:
         T.gather(d_src, d_idx, d_dst, n, s, variant=variant)
         torch.cuda.synchronize()
         dst2d = d_dst.view(n, k)
         for r0 in range(0, n, step):
             assert torch.equal(dst2d[r0:r0 + step], src2d[d_idx[r0:r0 + step].long()]), (variant, r0)
     back = torch.empty_like(d_src)
-    for variant in ("tile", "shfl"):
+    for variant in variants:
         T.scatter(d_src, d_idx, d_dst, n, s, variant=variant)
         T.gather(d_dst, d_idx, back, n, s, variant=variant)
         torch.cuda.synchronize()

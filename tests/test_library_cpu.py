@@ -63,6 +63,25 @@ def test_bench_kernels_limited_to_256_bytes():
         assert L.trove_load_sum(fake, fake, 100, b, 1, None) == T.ERR_SIZE
 
 
+def test_lds_variant_limits():
+    # The LDS-staged variant: whole wavefronts only (tile 0 or 64), struct
+    # sizes that are multiples of 4 up to 512 bytes, and not for the store /
+    # load benchmarks. Every call below returns before any launch.
+    L = T.lib()
+    fake = 0x10000
+    assert L.trove_copy(fake, fake, 10, 20, T.VARIANT_LDS, 32, None) == T.ERR_VARIANT
+    for b in (768, 1024, 2, 6):
+        assert not T.lds_supported(b)
+        assert L.trove_copy(fake, fake, 10, b, T.VARIANT_LDS, 64, None) == T.ERR_VARIANT
+        assert L.trove_gather(fake, fake, fake, 10, b, T.VARIANT_LDS, None) == T.ERR_VARIANT
+        assert L.trove_scatter(fake, fake, fake, 10, b, T.VARIANT_LDS, None) == T.ERR_VARIANT
+    assert L.trove_store_counting(fake, 10, 20, T.VARIANT_LDS, None) == T.ERR_VARIANT
+    assert L.trove_load_sum(fake, fake, 10, 20, T.VARIANT_LDS, None) == T.ERR_VARIANT
+    assert [b for b in T.STRUCT_SIZES if T.lds_supported(b)] == \
+        [b for b in T.STRUCT_SIZES if b % 4 == 0 and b <= 512]
+    assert all(T.word_bytes(b) == L.trove_struct_word_bytes(b) for b in T.STRUCT_SIZES)
+
+
 def test_validation_before_launch():
     L = T.lib()
     fake = 0x10000  # never dereferenced: every call below fails validation
@@ -78,6 +97,7 @@ def test_validation_before_launch():
     assert L.trove_gather(fake, fake + 2, fake, 10, 20, 3, None) == T.ERR_ALIGN
     assert L.trove_gather(fake, 0, fake, 10, 20, 3, None) == T.ERR_INVALID
     assert L.trove_scatter(fake, fake, fake, 10, 20, 9, None) == T.ERR_VARIANT
+    assert L.trove_gather(fake, fake, fake, 10, 20, T.VARIANT_LDS + 1, None) == T.ERR_VARIANT
     assert L.trove_gather(0, 0, 0, 0, 20, 3, None) == T.OK
     assert L.trove_store_counting(fake, 10, 6, 1, None) == T.ERR_SIZE
     assert L.trove_load_sum(fake, fake + 1, 10, 20, 1, None) == T.ERR_ALIGN

@@ -21,7 +21,7 @@ def test_choose_reads_the_table_and_falls_back():
     assert tune.choose(table, 20, "random", 1 << 19) == ("tile", 64)
     # No entry: tile path for multi-word structs, per-lane for one-word ones.
     assert tune.choose(table, 36, "runs", 5000) == ("tile", 16)
-    assert tune.choose(table, 16, "runs", 5000) == ("direct", 16)
+    assert tune.choose(table, 16, "runs", 5000) == ("tile", 16)
 
 
 def test_table_round_trip(tmp_path):
@@ -32,7 +32,10 @@ def test_table_round_trip(tmp_path):
     assert json.load(open(p))["version"] == 1
 
 
-def test_one_word_structs_only_try_the_per_lane_variant():
-    assert tune._variants(4) == ("direct",)
-    assert tune._variants(16) == ("direct",)
-    assert set(tune._variants(20)) == {"tile", "shfl", "direct"}
+def test_variant_candidates_per_size():
+    # One-word structs have no transposition: per-lane loads (tile policies
+    # or naive), or the LDS stage for dwordx4 stores; every other size tries
+    # all four paths.
+    assert tune._variants(4) == ("tile", "lds", "direct")
+    assert tune._variants(16) == ("tile", "lds", "direct")
+    assert set(tune._variants(20)) == {"tile", "shfl", "lds", "direct"}

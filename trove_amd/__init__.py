@@ -38,7 +38,24 @@ VARIANT_SHFL = 1
 VARIANT_DIRECT = 2
 VARIANT_TILE = 3
 
-VARIANTS = {"auto": VARIANT_AUTO, "shfl": VARIANT_SHFL, "direct": VARIANT_DIRECT, "tile": VARIANT_TILE}
+VARIANT_LDS = 4
+VARIANTS = {"auto": VARIANT_AUTO, "shfl": VARIANT_SHFL, "direct": VARIANT_DIRECT, "tile": VARIANT_TILE,
+            "lds": VARIANT_LDS}
+# The LDS-staged variant takes struct sizes that are multiples of 4 bytes up
+# to this many bytes (its stage is 64 structs per wavefront).
+LDS_MAX_STRUCT_BYTES = 512
+
+
+def lds_supported(struct_bytes: int) -> bool:
+    return struct_bytes % 4 == 0 and 4 <= struct_bytes <= LDS_MAX_STRUCT_BYTES
+
+
+def word_bytes(struct_bytes: int) -> int:
+    """Bytes of the word a struct moves in: the widest of 16, 8, 4 dividing it, else 1."""
+    for w in (16, 8, 4):
+        if struct_bytes % w == 0:
+            return w
+    return 1
 
 # Struct sizes compiled into the library (bytes).
 STRUCT_SIZES = tuple(range(4, 65, 4)) + (96, 128, 160, 192, 224, 256, 384, 512, 768, 1024, 2, 6)

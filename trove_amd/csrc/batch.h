@@ -22,8 +22,9 @@ struct job_desc {
 static_assert(sizeof(job_desc) == 56, "job_desc layout");
 
 // Launches the persistent gather over a device job table (sorted by
-// first_item, total_items work items in all).
-int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, hipStream_t s);
+// first_item, total_items work items in all). lds: some job is LDS-staged
+// (the launch then carries a 4 KiB LDS stage per wavefront).
+int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, bool lds, hipStream_t s);
 
 // Up to kArgJobs jobs travel in the kernel arguments instead (56 * 64 + 16
 // bytes, inside the 4 KiB argument segment): no table upload on the launch
@@ -35,6 +36,6 @@ struct job_pack {
     int32_t pad;
     int64_t total_items;
 };
-int launch_job_pack(const job_pack& p, hipStream_t s);
+int launch_job_pack(const job_pack& p, bool lds, hipStream_t s);
 
 }  // namespace trove_amd

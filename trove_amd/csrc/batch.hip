@@ -53,8 +53,30 @@ __device__ __forceinline__ P* assume_global(P* p) {
     return (P*)(global_p*)reinterpret_cast<uintptr_t>(p);
 }
 
+// LDS-staged copy items (load/store_warp_contiguous_lds); the partial last
+// group per lane.
+template <typename T>
+__device__ __forceinline__ void copy_range_lds(const T* __restrict__ src, T* __restrict__ dst, int64_t n, int64_t g0,
+                                               int64_t g1, uint32_t* stage) {
+    const int lane = trove::lane_id();
+    const int64_t full = n >> 6;
+    for (int64_t g = g0; g < g1; ++g) {
+        const int64_t i = (g << 6) + lane;
+        if (g < full) {
+            const T v = trove::load_warp_contiguous_lds<trove::streaming>(src + i, stage);
+            trove::store_warp_contiguous_lds<trove::streaming>(v, dst + i, stage);
+        } else if (i < n) {
+            trove::detail::divergent_store(trove::detail::divergent_load(src + i), dst + i);
+        }
+    }
+}
+
+// Per-wavefront LDS stage of the batch kernels: 64 structs of the largest
+// batch struct size (64 bytes), allocated only for tables with LDS jobs.
+constexpr int kBatchStageBytes = 64 * 64;
+
 template <int B>
-__device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local) {
+__device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local, uint32_t* stage) {
     typedef blob_t<B> T;
     job_desc J = J0;
     J.src = assume_global(J.src);
@@ -66,14 +88,22 @@ __device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local) {
     const int64_t g0 = local * J.chunk_groups;
     const int64_t g1 = g0 + J.chunk_groups < groups ? g0 + J.chunk_groups : groups;
     if (J.idx == nullptr) {
-        copy_range<T>(src, dst, J.n, g0, g1, J.variant != TROVE_VARIANT_DIRECT);
+        if (J.variant == TROVE_VARIANT_LDS) {
+            copy_range_lds<T>(src, dst, J.n, g0, g1, stage);
+        } else {
+            copy_range<T>(src, dst, J.n, g0, g1, J.variant != TROVE_VARIANT_DIRECT);
+        }
+        return;
+    }
+    if (J.variant == TROVE_VARIANT_LDS) {
+        gather_range_lds<T>(src, J.idx, dst, J.n, g0, g1, stage);
+        return;
+    }
+    if (J.variant == TROVE_VARIANT_TILE) {  // every batch size is a multiple of 4 bytes
+        gather_range_tile<T, gather_groups_in_flight<T>()>(src, J.idx, dst, J.n, g0, g1);
         return;
     }
     if constexpr (trove::detail::use_shfl<T, 64>::value) {
-        if (J.variant == TROVE_VARIANT_TILE) {
-            gather_range_tile<T, gather_groups_in_flight<T>()>(src, J.idx, dst, J.n, g0, g1);
-            return;
-        }
         if (J.variant == TROVE_VARIANT_SHFL) {  // items of one group (trove_batch_create)
             gather_group_shfl<T>(src, J.idx, dst, J.n, g0);
             return;
@@ -86,7 +116,7 @@ __device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local) {
 // by binary search on first_item (wave-uniform, scalar loads), then the
 // job's path for its struct size.
 template <typename Table>
-__device__ __forceinline__ void run_item(const Table& jobs, int n_jobs, int64_t it) {
+__device__ __forceinline__ void run_item(const Table& jobs, int n_jobs, int64_t it, uint32_t* stage) {
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -101,7 +131,7 @@ __device__ __forceinline__ void run_item(const Table& jobs, int n_jobs, int64_t 
     switch (J.bytes) {
 #define TROVE_BATCH_CASE(B)      \
     case B:                      \
-        batch_item<B>(J, local); \
+        batch_item<B>(J, local, stage); \
         break;
         TROVE_BATCH_SIZES(TROVE_BATCH_CASE)
 #undef TROVE_BATCH_CASE
@@ -113,27 +143,37 @@ __device__ __forceinline__ void run_item(const Table& jobs, int n_jobs, int64_t 
 // One work item per wavefront, no loop over items: in a loop, LICM hoists
 // the per-lane tables of every (size, variant) branch and keeps them all
 // live (275 VGPRs). The grid has exactly total_items wavefronts.
+// The wavefront's slice of the dynamic LDS (present when the launch has LDS
+// jobs; otherwise no item touches it).
+__device__ __forceinline__ uint32_t* batch_stage() {
+    extern __shared__ __align__(16) uint32_t lds_stage[];
+    return lds_stage + (threadIdx.x >> 6) * (kBatchStageBytes / 4);
+}
+
 __global__ __launch_bounds__(kBlock) void k_batch_gather(const job_desc* __restrict__ jobs, int n_jobs,
                                                          int64_t total_items) {
     const int64_t it = wave_index();
-    if (it < total_items) run_item(jobs, n_jobs, it);
+    if (it < total_items) run_item(jobs, n_jobs, it, batch_stage());
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_gather_pack(const job_pack p) {
     const int64_t it = wave_index();
-    if (it < p.total_items) run_item(p.jobs, p.n_jobs, it);
+    if (it < p.total_items) run_item(p.jobs, p.n_jobs, it, batch_stage());
 }
 
-int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, hipStream_t s) {
+int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, bool lds, hipStream_t s) {
     // One wavefront per item: never capped.
     const int64_t blocks = (total_items + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(k_batch_gather, dim3((unsigned)blocks), dim3(kBlock), 0, s, d_table, n_jobs, total_items);
+    const size_t stage = lds ? (size_t)kBatchStageBytes * kWavesPerBlock : 0;
+    hipLaunchKernelGGL(k_batch_gather, dim3((unsigned)blocks), dim3(kBlock), stage, s, d_table, n_jobs,
+                       total_items);
     return launch_status();
 }
 
-int launch_job_pack(const job_pack& p, hipStream_t s) {
+int launch_job_pack(const job_pack& p, bool lds, hipStream_t s) {
     const int64_t blocks = (p.total_items + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(k_batch_gather_pack, dim3((unsigned)blocks), dim3(kBlock), 0, s, p);
+    const size_t stage = lds ? (size_t)kBatchStageBytes * kWavesPerBlock : 0;
+    hipLaunchKernelGGL(k_batch_gather_pack, dim3((unsigned)blocks), dim3(kBlock), stage, s, p);
     return launch_status();
 }
 
@@ -147,6 +187,7 @@ struct trove_batch {
     job_desc* d_table = nullptr;
     int64_t total_items = 0;
     int64_t algorithmic_bytes = 0;
+    bool lds = false;  // some job takes the LDS-staged path
     int device = 0;
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -154,7 +195,7 @@ struct trove_batch {
 };
 
 static int batch_launch(trove_batch* b, hipStream_t s) {
-    return launch_job_table(b->d_table, (int)b->table.size(), b->total_items, s);
+    return launch_job_table(b->d_table, (int)b->table.size(), b->total_items, b->lds, s);
 }
 
 extern "C" {
@@ -177,7 +218,7 @@ int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out)
             delete b;
             return TROVE_ERR_INVALID;
         }
-        if (J.variant < TROVE_VARIANT_AUTO || J.variant > TROVE_VARIANT_TILE || J.chunk_groups < 0) {
+        if (J.variant < TROVE_VARIANT_AUTO || J.variant > TROVE_VARIANT_LDS || J.chunk_groups < 0) {
             delete b;
             return TROVE_ERR_VARIANT;
         }
@@ -194,15 +235,19 @@ int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out)
         d.bytes = bytes;
         const bool copy = J.idx == nullptr;
         if (copy) {
-            // A copy job: SHFL (load/store_warp_contiguous) unless DIRECT.
-            d.variant = J.variant == TROVE_VARIANT_DIRECT ? TROVE_VARIANT_DIRECT : TROVE_VARIANT_SHFL;
+            // A copy job: SHFL (load/store_warp_contiguous), LDS
+            // (load/store_warp_contiguous_lds) or DIRECT; AUTO takes the
+            // library's per-size choice, TILE means SHFL.
+            const int v = J.variant == TROVE_VARIANT_AUTO ? auto_copy_variant(bytes) : J.variant;
+            d.variant = v == TROVE_VARIANT_TILE ? TROVE_VARIANT_SHFL : v;
         } else {
-            d.variant = J.variant == TROVE_VARIANT_AUTO ? TROVE_VARIANT_TILE : J.variant;
+            d.variant = J.variant == TROVE_VARIANT_AUTO ? auto_gather_variant(bytes) : J.variant;
         }
         d.chunk_groups = J.chunk_groups > 0 ? J.chunk_groups : 16;
         // coalesced_ptr gather items are one group: a loop around the tile
         // cascade would keep every level's tables live.
         if (!copy && d.variant == TROVE_VARIANT_SHFL) d.chunk_groups = 1;
+        b->lds = b->lds || d.variant == TROVE_VARIANT_LDS;
         d.first_item = item;
         const int64_t groups = (J.n + 63) / 64;
         item += (groups + d.chunk_groups - 1) / d.chunk_groups;

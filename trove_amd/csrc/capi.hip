@@ -95,7 +95,7 @@ static const size_entry* find_size(int bytes) {
     return nullptr;
 }
 
-static bool valid_variant(int v) { return v >= TROVE_VARIANT_AUTO && v <= TROVE_VARIANT_TILE; }
+static bool valid_variant(int v) { return v >= TROVE_VARIANT_AUTO && v <= TROVE_VARIANT_LDS; }
 
 // Largest n the one-unit-per-wavefront kernels cover with a grid of at most
 // 2^31 - 1 single-wavefront work-groups (2^37 structs, far beyond any HBM).
@@ -139,6 +139,7 @@ int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int vari
     if (n < 0 || n > kMaxStructs) return TROVE_ERR_INVALID;
     if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE) return TROVE_ERR_VARIANT;
     if (tile != 0 && (tile < 2 || tile > 64 || (tile & (tile - 1)) != 0)) return TROVE_ERR_VARIANT;
+    if (variant == TROVE_VARIANT_LDS && (tile != 0 && tile != 64)) return TROVE_ERR_VARIANT;  // whole wavefronts
     if (n == 0) return TROVE_OK;
     if (!src || !dst) return TROVE_ERR_INVALID;
     const int w = word_bytes_of(struct_bytes);
@@ -184,7 +185,8 @@ int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, vo
     const size_entry* e = find_size(struct_bytes);
     if (!e || struct_bytes % 4 != 0) return TROVE_ERR_SIZE;
     if (n < 0) return TROVE_ERR_INVALID;
-    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE) return TROVE_ERR_VARIANT;
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS)
+        return TROVE_ERR_VARIANT;
     if (n == 0) return TROVE_OK;
     if (!dst) return TROVE_ERR_INVALID;
     if (!aligned(dst, word_bytes_of(struct_bytes))) return TROVE_ERR_ALIGN;
@@ -195,7 +197,8 @@ int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, 
     const size_entry* e = find_size(struct_bytes);
     if (!e || struct_bytes % 4 != 0) return TROVE_ERR_SIZE;
     if (n < 0) return TROVE_ERR_INVALID;
-    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE) return TROVE_ERR_VARIANT;
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS)
+        return TROVE_ERR_VARIANT;
     if (n == 0) return TROVE_OK;
     if (!src || !out) return TROVE_ERR_INVALID;
     if (!aligned(src, word_bytes_of(struct_bytes)) || !aligned(out, 4)) return TROVE_ERR_ALIGN;

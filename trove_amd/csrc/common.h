@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <trove/aos.h>
 #include <trove/block.h>
+#include <trove/lds.h>
 #include <trove/ptr.h>
 #include <trove_amd.h>
 
@@ -13,6 +14,32 @@ namespace trove_amd {
 
 // Work-group size of every kernel: four wavefronts.
 constexpr int kBlock = 256;
+// Largest struct of the LDS-staged variant: its stage is 64 * sizeof(T)
+// bytes per wavefront (32 KiB at 512 bytes, 5 wavefronts per CU), and above
+// 512 bytes the struct no longer fits the registers beside its chunks.
+constexpr int kMaxLdsStructBytes = 512;
+
+// TROVE_VARIANT_AUTO per struct size, from interleaved A/B timings on MI355X
+// (median of 15 launches, 2^26 structs up to 64 bytes, 2^24 up to 256, 2^22
+// above) confirmed by a second, independent bench run: the LDS-staged path
+// only where it was faster in both. Copies: the structs of at most 16 bytes
+// (a full dwordx4 per lane instead of narrower per-lane words) and the
+// 16-byte-word structs of 32 and 96-160 bytes (no two-rotation schedule);
+// gathers: 16, 48, 64 and 192 bytes; scatters: 8 and 16 bytes. 4-byte
+// scatters keep the naive kernel (the streaming one-word path runs at half
+// its rate). Everything else takes the transposition (copy) or the tile path
+// (gather, scatter).
+constexpr int auto_copy_variant(int b) {
+    return (b == 4 || b == 8 || b == 12 || b == 16 || b == 32 || b == 96 || b == 128 || b == 160)
+               ? TROVE_VARIANT_LDS
+               : TROVE_VARIANT_SHFL;
+}
+constexpr int auto_gather_variant(int b) {
+    return (b == 16 || b == 48 || b == 64 || b == 192) ? TROVE_VARIANT_LDS : TROVE_VARIANT_TILE;
+}
+constexpr int auto_scatter_variant(int b) {
+    return b == 4 ? TROVE_VARIANT_DIRECT : (b == 8 || b == 16) ? TROVE_VARIANT_LDS : TROVE_VARIANT_TILE;
+}
 constexpr int kWavesPerBlock = kBlock / 64;
 
 // The struct type moved for a given byte size: k 32-bit words when the size

@@ -2,6 +2,8 @@
 //  SHFL   - one struct per lane through load_warp_contiguous /
 //           store_warp_contiguous<TILE> (the Trove low-level API), with the
 //           streaming (nontemporal) cache policy.
+//  LDS    - the LDS-staged transposition (load/store_warp_contiguous_lds):
+//           dwordx4 tile traffic through a per-wavefront LDS stage.
 //  DIRECT - the naive per-lane struct copy the compiler generates.
 // Also the reference benchmark's store-only (counting structs) and load-only
 // (per-struct word sum) kernels, tests/benchmark.cu:52-88.
@@ -66,6 +68,50 @@ __global__ __launch_bounds__(B) void k_copy_shfl(const T* __restrict__ src, T* _
     }
 }
 
+// LDS-staged variant (trove/lds.h): the same one-shot grid of one-wavefront
+// work-groups, each staging its UNROLL groups (UNROLL * 64 * sizeof(T)
+// contiguous bytes) through as many bytes of dynamic LDS. The unit moves as
+// ONE chunk array, so that small structs (UNROLL > 1) still give every lane a
+// full chunk: at 4 bytes one dwordx4 per lane instead of four 16-lane ones.
+// CW is the chunk width (4 dwords when src and dst are 16-byte aligned, else
+// 1). Struct l of each group passes through lane l's registers between the
+// two stagings, as with load/store_warp_contiguous_lds.
+template <int W>
+struct unit_words {
+    uint32_t w[W];
+};
+
+template <typename T, int UNROLL, int CW>
+__global__ __launch_bounds__(64) void k_copy_lds(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
+    extern __shared__ __align__(16) uint32_t lds_stage[];
+    namespace d = trove::detail;
+    constexpr int k = (int)(sizeof(T) / 4);
+    typedef unit_words<UNROLL * k> Unit;  // 64 units = the wavefront's UNROLL groups
+    const int lane = trove::lane_id();
+    const int64_t groups = n >> 6;
+    const int64_t g0 = (int64_t)blockIdx.x * UNROLL;
+    if (g0 + UNROLL <= groups) {
+        d::lds_chunks<Unit, CW> t;
+        d::lds_global_load<CW, trove::streaming>(reinterpret_cast<const Unit*>(src + (g0 << 6)), lane, t);
+        d::lds_put_chunks(lds_stage, lane, t);
+        d::wave_lds_sync();
+        T v[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) v[u] = d::lds_get_struct<T>(lds_stage + u * 64 * k, lane);
+        d::wave_lds_sync();
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) d::lds_put_struct(lds_stage + u * 64 * k, lane, v[u]);
+        d::wave_lds_sync();
+        d::lds_get_chunks(lds_stage, lane, t);
+        d::lds_global_store<CW, trove::streaming>(reinterpret_cast<Unit*>(dst + (g0 << 6)), lane, t);
+    } else {
+        for (int64_t g = g0; g < groups; ++g) {
+            const T v = trove::load_warp_contiguous_lds<trove::streaming>(src + (g << 6) + lane, lds_stage);
+            trove::store_warp_contiguous_lds<trove::streaming>(v, dst + (g << 6) + lane, lds_stage);
+        }
+    }
+}
+
 // Partial trailing group (n % 64 structs), one wavefront: the active lanes go
 // through the tile cascade. A kernel of its own, so that the streaming kernel
 // does not carry the cascade's registers (up to 512 per lane at 1 KiB).
@@ -100,12 +146,46 @@ static int launch_copy_shfl(const T* src, T* dst, int64_t n, hipStream_t s) {
     return TROVE_OK;
 }
 
+template <typename T>
+static int launch_copy_lds(const T* src, T* dst, int64_t n, hipStream_t s) {
+    constexpr int words = (int)(sizeof(T) / 4);
+    constexpr int unroll = words < 4 ? 4 / words : 1;
+    const int64_t groups = n >> 6;
+    if (groups > 0) {
+        const int64_t blocks = grid_for((groups + unroll - 1) / unroll, 1);
+        const size_t stage = (size_t)unroll * 64 * sizeof(T);
+        const size_t window = stream_lds_reservation((int64_t)stage);
+        const size_t lds = window > stage ? window : stage;
+        if (aligned(src, 16) && aligned(dst, 16)) {
+            hipLaunchKernelGGL((k_copy_lds<T, unroll, 4>), dim3((unsigned)blocks), dim3(64), lds, s, src, dst, n);
+        } else {
+            hipLaunchKernelGGL((k_copy_lds<T, unroll, 1>), dim3((unsigned)blocks), dim3(64), lds, s, src, dst, n);
+        }
+        const int rc = launch_status();
+        if (rc != TROVE_OK) return rc;
+    }
+    if (n & 63) {
+        hipLaunchKernelGGL((k_copy_tail<T>), dim3(1), dim3(64), 0, s, src, dst, n);
+        return launch_status();
+    }
+    return TROVE_OK;
+}
+
 template <int B>
 int copy_impl(const void* src_, void* dst_, int64_t n, int variant, int tile, hipStream_t s) {
     typedef blob_t<B> T;
     const T* src = static_cast<const T*>(src_);
     T* dst = static_cast<T*>(dst_);
-    if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL) {
+    if (variant == TROVE_VARIANT_AUTO)  // a sub-wavefront tile asks for the transposition
+        variant = (tile == 0 || tile == 64) ? auto_copy_variant(B) : TROVE_VARIANT_SHFL;
+    if (variant == TROVE_VARIANT_LDS) {
+        if constexpr (B % 4 == 0 && B <= kMaxLdsStructBytes) {
+            return launch_copy_lds<T>(src, dst, n, s);
+        } else {
+            return TROVE_ERR_VARIANT;
+        }
+    }
+    if (variant == TROVE_VARIANT_SHFL) {
         switch (tile == 0 ? 64 : tile) {
             case 64: return launch_copy_shfl<T, 64>(src, dst, n, s);
             case 32: return launch_copy_shfl<T, 32>(src, dst, n, s);

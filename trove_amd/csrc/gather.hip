@@ -7,6 +7,9 @@
 //  TILE   - the same tile-indexed word accesses with the transposition
 //           elided: the gathered words already sit in the column-side layout
 //           a coalesced tile store takes (and scatter mirrors it).
+//  LDS    - the coalesced_ptr kernels' LDS-staged counterpart: tile-indexed
+//           word accesses, one struct per lane through an LDS stage, chunked
+//           (dwordx4) accesses on the contiguous side (trove/lds.h).
 //  DIRECT - the naive per-lane kernels (tests/benchmark.cu:112-128).
 #include "gather_body.h"
 
@@ -32,6 +35,56 @@ __global__ __launch_bounds__(kBlock) void k_gather_shfl(const T* __restrict__ sr
                                                         T* __restrict__ dst, int64_t n) {
     const int64_t g = wave_index();
     if (g < ((n + 63) >> 6)) gather_group_shfl<T>(src, idx, dst, n, g);
+}
+
+// LDS-staged gather: one group per one-wavefront work-group (64 * sizeof(T)
+// bytes of dynamic LDS, up to 64 KiB at 1 KiB structs); the trailing partial
+// group per lane.
+template <typename T, int CW>
+__global__ __launch_bounds__(64) void k_gather_lds(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                   T* __restrict__ dst, int64_t n) {
+    extern __shared__ __align__(16) uint32_t lds_stage[];
+    const int64_t g = blockIdx.x;
+    if (g < (n >> 6)) {
+        gather_group_lds<T, CW>(src, idx, dst, g, lds_stage);
+    } else {
+        const int64_t i = (g << 6) + trove::lane_id();
+        if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + idx[i]), dst + i);
+    }
+}
+
+// LDS-staged scatter: CW-dword chunk loads of the group through the stage to
+// one struct per lane (load_warp_contiguous_lds), then back through the stage
+// to column-side words for the tile-indexed stores (m = 1: per-lane stores).
+template <typename T, int CW>
+__global__ __launch_bounds__(64) void k_scatter_lds(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                    T* __restrict__ dst, int64_t n) {
+    extern __shared__ __align__(16) uint32_t lds_stage[];
+    namespace d = trove::detail;
+    typedef d::word_layout<T> L;
+    const int64_t g = blockIdx.x;
+    const int lane = trove::lane_id();
+    const int64_t i = (g << 6) + lane;
+    if (g >= (n >> 6)) {
+        if (i < n) d::divergent_store(d::divergent_load(src + i), dst + idx[i]);
+        return;
+    }
+    const int32_t ix = d::load_word<trove::streaming>(idx + i);
+    d::lds_chunks<T, CW> t;
+    d::lds_global_load<CW, trove::streaming>(src + (g << 6), lane, t);
+    d::lds_put_chunks(lds_stage, lane, t);
+    d::wave_lds_sync();
+    const T v = d::lds_get_struct<T>(lds_stage, lane);
+    if constexpr (L::m == 1) {
+        d::divergent_store(v, dst + ix);
+    } else {
+        d::wave_lds_sync();
+        d::lds_put_struct(lds_stage, lane, v);
+        d::wave_lds_sync();
+        uint32_t w[L::m * L::W];
+        d::lds_get_columns<T>(lds_stage, lane, w);
+        d::tile_scatter_indexed<64>(dst, ix, lane, 0, w);
+    }
 }
 
 template <typename T>
@@ -60,6 +113,29 @@ __global__ __launch_bounds__(kBlock) void k_scatter_tile(const T* __restrict__ s
     const int64_t groups = n >> 6;
     const int64_t step = wave_count() * UNROLL;
     int64_t g = wave_index() * UNROLL;
+    if constexpr (L::m == 1) {
+        // One-word structs: streaming contiguous loads, per-lane row stores.
+        typedef typename L::U U;
+        const U* s = reinterpret_cast<const U*>(src);
+        U* d = reinterpret_cast<U*>(dst);
+        for (; g + UNROLL <= groups; g += step) {
+            int32_t ix[UNROLL];
+            U v[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                ix[u] = trove::detail::load_word<trove::streaming>(idx + ((g + u) << 6) + lane);
+                v[u] = trove::detail::load_word<trove::streaming>(s + ((g + u) << 6) + lane);
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) trove::detail::store_word<trove::cached>(d + ix[u], v[u]);
+        }
+        for (; g < groups; ++g) d[idx[(g << 6) + lane]] = s[(g << 6) + lane];
+        if (wave_index() == 0 && (n & 63)) {
+            const int64_t i = (groups << 6) + lane;
+            if (i < n) d[idx[i]] = s[i];
+        }
+        return;
+    }
     for (; g + UNROLL <= groups; g += step) {
         int32_t ix[UNROLL];
         uint32_t v[UNROLL][L::m * L::W];
@@ -89,16 +165,38 @@ __global__ __launch_bounds__(kBlock) void k_scatter_direct(const T* __restrict__
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[idx[i]] = src[i];
 }
 
+// Structs the TILE variant covers: the shuffle-eligible sizes and the
+// one-word structs of 4, 8 and 16 bytes (per-lane row words).
+template <typename T>
+constexpr bool tile_path() {
+    return trove::detail::use_shfl<T, 64>::value ||
+           (sizeof(T) % 4 == 0 && trove::detail::word_layout<T>::m == 1);
+}
+
 template <int B>
 int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int variant, hipStream_t s) {
     typedef blob_t<B> T;
     const T* src = static_cast<const T*>(src_);
     T* dst = static_cast<T*>(dst_);
-    constexpr bool tiled = trove::detail::use_shfl<T, 64>::value;
+    constexpr bool tiled = tile_path<T>();
     constexpr int U = gather_groups_in_flight<T>();
     const int64_t units = ((n + 63) / 64 + U - 1) / U;
-    if (variant == TROVE_VARIANT_AUTO) variant = tiled ? TROVE_VARIANT_TILE : TROVE_VARIANT_DIRECT;
+    if (variant == TROVE_VARIANT_AUTO) variant = auto_gather_variant(B);
     if (variant == TROVE_VARIANT_TILE && !tiled) variant = TROVE_VARIANT_DIRECT;
+    if (variant == TROVE_VARIANT_LDS) {
+        if constexpr (B % 4 == 0 && B <= kMaxLdsStructBytes) {
+            const unsigned blocks = (unsigned)grid_for((n + 63) / 64, 1);
+            const size_t lds = trove::lds_stage_bytes<T>();
+            if (aligned(dst, 16)) {
+                hipLaunchKernelGGL((k_gather_lds<T, 4>), dim3(blocks), dim3(64), lds, s, src, idx, dst, n);
+            } else {
+                hipLaunchKernelGGL((k_gather_lds<T, 1>), dim3(blocks), dim3(64), lds, s, src, idx, dst, n);
+            }
+            return launch_status();
+        } else {
+            return TROVE_ERR_VARIANT;
+        }
+    }
     if (variant == TROVE_VARIANT_SHFL) {
         hipLaunchKernelGGL((k_gather_shfl<T>), dim3((unsigned)grid_for((n + 63) / 64)), dim3(kBlock), 0, s, src,
                            idx, dst, n);
@@ -124,11 +222,25 @@ int scatter_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, in
     typedef blob_t<B> T;
     const T* src = static_cast<const T*>(src_);
     T* dst = static_cast<T*>(dst_);
-    constexpr bool tiled = trove::detail::use_shfl<T, 64>::value;
+    constexpr bool tiled = tile_path<T>();
     constexpr int U = gather_groups_in_flight<T>();
     const int64_t units = ((n >> 6) + U - 1) / U;
-    if (variant == TROVE_VARIANT_AUTO) variant = tiled ? TROVE_VARIANT_TILE : TROVE_VARIANT_DIRECT;
+    if (variant == TROVE_VARIANT_AUTO) variant = auto_scatter_variant(B);
     if (variant == TROVE_VARIANT_TILE && !tiled) variant = TROVE_VARIANT_DIRECT;
+    if (variant == TROVE_VARIANT_LDS) {
+        if constexpr (B % 4 == 0 && B <= kMaxLdsStructBytes) {
+            const unsigned blocks = (unsigned)grid_for((n + 63) / 64, 1);
+            const size_t lds = trove::lds_stage_bytes<T>();
+            if (aligned(src, 16)) {
+                hipLaunchKernelGGL((k_scatter_lds<T, 4>), dim3(blocks), dim3(64), lds, s, src, idx, dst, n);
+            } else {
+                hipLaunchKernelGGL((k_scatter_lds<T, 1>), dim3(blocks), dim3(64), lds, s, src, idx, dst, n);
+            }
+            return launch_status();
+        } else {
+            return TROVE_ERR_VARIANT;
+        }
+    }
     if (variant == TROVE_VARIANT_SHFL) {
         hipLaunchKernelGGL((k_scatter_shfl<T>), dim3((unsigned)grid_for((n + 63) / 64)), dim3(kBlock), 0, s, src,
                            idx, dst, n);

@@ -22,15 +22,54 @@ constexpr int gather_groups_in_flight() {
 
 // Tile path: indexed word loads in column-side layout (each word's owner
 // index broadcast by ds_bpermute), then coalesced tile stores; no
-// transposition. UNROLL groups are in flight per iteration.
+// transposition. UNROLL groups are in flight per iteration. Structs of 4, 8
+// or 16 bytes take gather_range_word.
 // The index stream and the destination are touched once (streaming policy);
 // the gathered source keeps ordinary loads (nontemporal loads of random rows
 // measured 20-30% slower on MI355X).
+// One-word structs (m = 1: 4, 8 and 16 bytes) have nothing to broadcast or
+// transpose: each lane loads its row as one word. They keep the tile path's
+// policies (streaming index loads and stores, UNROLL groups in flight).
+template <typename T, int UNROLL>
+__device__ __forceinline__ void gather_range_word(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                  T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1) {
+    typedef typename trove::detail::word_layout<T>::U U;
+    typedef trove::streaming S;
+    const U* s = reinterpret_cast<const U*>(src);
+    U* d = reinterpret_cast<U*>(dst);
+    const int lane = trove::lane_id();
+    const int64_t full = n >> 6;
+    const int64_t full_end = g1 < full ? g1 : full;
+    int64_t g = g0;
+    for (; g + UNROLL <= full_end; g += UNROLL) {
+        int32_t ix[UNROLL];
+        U v[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) ix[u] = trove::detail::load_word<S>(idx + ((g + u) << 6) + lane);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) v[u] = trove::detail::load_word<trove::cached>(s + ix[u]);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) trove::detail::store_word<S>(d + ((g + u) << 6) + lane, v[u]);
+    }
+    for (; g < full_end; ++g) {
+        const int64_t i = (g << 6) + lane;
+        trove::detail::store_word<S>(d + i, trove::detail::load_word<trove::cached>(s + trove::detail::load_word<S>(idx + i)));
+    }
+    if (g1 > full && (n & 63)) {
+        const int64_t i = (full << 6) + lane;
+        if (i < n) d[i] = s[idx[i]];
+    }
+}
+
 template <typename T, int UNROLL>
 __device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
                                                   T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1) {
     typedef trove::detail::word_layout<T> L;
     typedef trove::streaming S;
+    if constexpr (L::m == 1) {
+        gather_range_word<T, UNROLL>(src, idx, dst, n, g0, g1);
+        return;
+    }
     const int lane = trove::lane_id();
     const int64_t full = n >> 6;
     const int64_t full_end = g1 < full ? g1 : full;
@@ -69,6 +108,56 @@ __device__ __forceinline__ void gather_group_shfl(const T* __restrict__ src, con
         trove::coalesced_ptr<T> r(dst);
         const T v = s[idx[i]];
         r[i] = v;
+    }
+}
+
+// LDS-staged path (trove/lds.h) for ONE full group g, the LDS counterpart of
+// the coalesced_ptr path: the tile path's indexed loads (column-side words)
+// go through the stage so that lane l holds struct l, as a coalesced_ptr read
+// delivers it, then back through the stage to CW-dword chunk stores. stage:
+// 64 * sizeof(T) bytes of LDS private to the wavefront. One-word structs
+// (m = 1) load their row per lane and only stage the store.
+template <typename T, int CW>
+__device__ __forceinline__ void gather_group_lds(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                 T* __restrict__ dst, int64_t g, uint32_t* stage) {
+    namespace d = trove::detail;
+    typedef d::word_layout<T> L;
+    const int lane = trove::lane_id();
+    const int32_t ix = d::load_word<trove::streaming>(idx + (g << 6) + lane);
+    T v;
+    if constexpr (L::m == 1) {
+        v = d::divergent_load(src + ix);
+    } else {
+        uint32_t w[L::m * L::W];
+        d::tile_gather_indexed<64>(src, ix, lane, 0, w);
+        d::wave_lds_sync();
+        d::lds_put_columns<T>(stage, lane, w);
+        d::wave_lds_sync();
+        v = d::lds_get_struct<T>(stage, lane);
+    }
+    d::wave_lds_sync();
+    d::lds_put_struct(stage, lane, v);
+    d::wave_lds_sync();
+    d::lds_chunks<T, CW> t;
+    d::lds_get_chunks(stage, lane, t);
+    d::lds_global_store<CW, trove::streaming>(dst + (g << 6), lane, t);
+}
+
+// LDS-staged gather over groups [g0, g1); the partial last group per lane.
+template <typename T>
+__device__ __forceinline__ void gather_range_lds(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                 T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1,
+                                                 uint32_t* stage) {
+    const int64_t full = n >> 6;
+    const int64_t full_end = g1 < full ? g1 : full;
+    if (trove::detail::lds_wide(dst)) {
+        for (int64_t g = g0; g < full_end; ++g) gather_group_lds<T, 4>(src, idx, dst, g, stage);
+    } else {
+        for (int64_t g = g0; g < full_end; ++g) gather_group_lds<T, 1>(src, idx, dst, g, stage);
+    }
+    if (g1 > full && (n & 63)) {
+        const int64_t i = (full << 6) + trove::lane_id();
+        if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + idx[i]), dst + i);
     }
 }
 

@@ -250,10 +250,10 @@ static int launch_step(trove_serve* s) {
         p.n_jobs = nj;
         p.pad = 0;
         p.total_items = items;
-        rc = launch_job_pack(p, s->stream);
+        rc = launch_job_pack(p, s->variant == TROVE_VARIANT_LDS, s->stream);
     } else {
         rc = hip_status(hipMemcpyAsync(s->d_jobs[k], jobs, sizeof(job_desc) * nj, hipMemcpyHostToDevice, s->stream));
-        if (rc == TROVE_OK) rc = launch_job_table(s->d_jobs[k], nj, items, s->stream);
+        if (rc == TROVE_OK) rc = launch_job_table(s->d_jobs[k], nj, items, s->variant == TROVE_VARIANT_LDS, s->stream);
     }
     if (rc == TROVE_OK) rc = hip_status(hipEventRecord(st.done, s->stream));
     if (rc != TROVE_OK) return rc;
@@ -287,9 +287,7 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
     if (!table || n_rows <= 0 || n_rows > INT32_MAX) return TROVE_ERR_INVALID;
     if (row_bytes < 4 || row_bytes > 64 || row_bytes % 4 != 0) return TROVE_ERR_SIZE;
     if (budget_bytes < row_bytes || max_slots <= 0 || slice_indices < 64) return TROVE_ERR_INVALID;
-    if (variant != TROVE_VARIANT_AUTO && variant != TROVE_VARIANT_TILE && variant != TROVE_VARIANT_SHFL &&
-        variant != TROVE_VARIANT_DIRECT)
-        return TROVE_ERR_VARIANT;
+    if (variant < TROVE_VARIANT_AUTO || variant > TROVE_VARIANT_LDS) return TROVE_ERR_VARIANT;
     if (!aligned(table, word_bytes_of(row_bytes))) return TROVE_ERR_ALIGN;
     trove_serve* s = new (std::nothrow) trove_serve();
     if (!s) return TROVE_ERR_NOMEM;
@@ -299,7 +297,7 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
     s->capacity_rows = budget_bytes / row_bytes;
     s->max_slots = max_slots;
     s->slice = slice_indices;
-    s->variant = variant == TROVE_VARIANT_AUTO ? TROVE_VARIANT_TILE : variant;
+    s->variant = variant == TROVE_VARIANT_AUTO ? auto_gather_variant(row_bytes) : variant;
     s->stream = static_cast<hipStream_t>(stream);
     s->max_jobs = max_slots;
     s->arena = new (std::nothrow) range_allocator(s->capacity_rows);

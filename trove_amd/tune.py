@@ -1,7 +1,8 @@
 """Autotuner for batched gathers.
 
 Picks, per (struct size, index pattern, length bucket), the kernel variant
-(tile path, coalesced_ptr path, naive per-lane loads) and the work-item size
+(tile path, coalesced_ptr path with the shuffle transposition, its LDS-staged
+counterpart, naive per-lane loads) and the work-item size
 (64-struct groups per wavefront item) of the persistent batch kernel, by
 timing single-job batches of that shape on the device with HIP events. The
 table is plain JSON so a tuned device can be reused without re-timing.
@@ -27,9 +28,10 @@ def bucket(n: int, lengths=LENGTHS) -> int:
 
 
 def _variants(size: int):
-    # One-word structs (4, 8, 16 B) have no transposition: tile and
-    # coalesced_ptr paths reduce to per-lane accesses.
-    return ("direct",) if size in (4, 8, 16) else ("tile", "shfl", "direct")
+    # One-word structs (4, 8, 16 B) have no transposition: the coalesced_ptr
+    # path reduces to per-lane accesses (the tile path keeps its streaming
+    # policies, the LDS-staged path turns the stores into dwordx4 chunks).
+    return ("tile", "lds", "direct") if size in (4, 8, 16) else ("tile", "shfl", "lds", "direct")
 
 
 def autotune(sizes=SIZES, patterns=PATTERNS, lengths=LENGTHS, reps: int = 3, seed: int = 1, log=None) -> dict:
@@ -49,7 +51,7 @@ def autotune(sizes=SIZES, patterns=PATTERNS, lengths=LENGTHS, reps: int = 3, see
             for s in sizes:
                 best = None
                 for variant in _variants(s):
-                    for chunk in (CHUNKS if variant != "shfl" else (1,)):
+                    for chunk in (CHUNKS if variant != "shfl" else (1,)):  # shfl items are one group
                         b = T.Batch([(src, idx, dst, L, s, variant, chunk)])
                         b.run("persistent")  # warm-up
                         times = []
@@ -90,4 +92,4 @@ def choose(table: dict, size: int, pattern: str, n: int):
     for e in table["entries"]:
         if e["bytes"] == size and e["pattern"] == pattern and e["length"] == L:
             return e["variant"], e["chunk_groups"]
-    return ("direct" if size in (4, 8, 16) else "tile"), 16
+    return "tile", 16
