@@ -4,6 +4,7 @@
 // include/trove for gfx950.
 #include <trove/aos.h>
 #include <trove/block.h>
+#include <trove/lds.h>
 #include <trove/ptr.h>
 
 template <typename T>
@@ -63,6 +64,17 @@ __global__ void transpose_api(T* p) {
     p[threadIdx.x] = trove::get<2>(trove::rotate(v, threadIdx.x & 3));
 }
 
+// The LDS-staged addition (trove/lds.h) in a user kernel: four wavefronts
+// per work-group, each with its own stage.
+template <typename T>
+__global__ __launch_bounds__(256) void lds_staged_copy(const T* src, T* dst) {
+    __shared__ __align__(16) uint32_t stage[4 * trove::lds_stage_bytes<T>() / 4];
+    uint32_t* mine = stage + (threadIdx.x / 64) * (trove::lds_stage_bytes<T>() / 4);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    T v = trove::load_warp_contiguous_lds<trove::streaming>(src + i, mine);
+    trove::store_warp_contiguous_lds(v, dst + i, mine);
+}
+
 struct odd_struct {
     char c[6];
 };
@@ -82,3 +94,6 @@ template __global__ void ref_to_ref<trove::array<int, 6> >(const int*, trove::ar
                                                            trove::array<int, 6>*);
 template __global__ void transpose_api<int>(int*);
 template __global__ void transpose_api<double>(double*);
+template __global__ void lds_staged_copy<trove::array<int, 5> >(const trove::array<int, 5>*, trove::array<int, 5>*);
+template __global__ void lds_staged_copy<trove::array<float, 8> >(const trove::array<float, 8>*,
+                                                                 trove::array<float, 8>*);
