@@ -413,7 +413,6 @@ def main():
                            lambda: T.gather(src, idx, dst, n, s, variant="direct"))
                 # The LDS-staged variant (trove/lds.h) beside the transposition.
                 nclock.run(f"copy_lds_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="lds"))
-                nclock.run(f"copy_auto_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="auto"))
                 nclock.run(f"gather_lds_k{k}", (2 * s + 4) * n, lambda: T.gather(src, idx, dst, n, s, variant="lds"))
             for _ in range(2):
                 nclock.run(f"scatter_tile_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="tile"))
@@ -471,9 +470,12 @@ def main():
                         "scatter_naive_gbs": naive[f"scatter_direct_k{k}"]["gbs"]})
             row["lds_gbs"] = {"copy": naive[f"copy_lds_k{k}"]["gbs"], "gather": naive[f"gather_lds_k{k}"]["gbs"],
                               "scatter": naive[f"scatter_lds_k{k}"]["gbs"]}
-            # The library's per-size choice (LDS-staged or register transposition).
-            row["copy_auto_gbs"] = naive[f"copy_auto_k{k}"]["gbs"]
-            row["copy_auto_frac_peak"] = round(row["copy_auto_gbs"] / HBM_PEAK_GBS, 3)
+            # The path AUTO takes at this size (LDS-staged or register
+            # transposition) and its rate from that path's own leg above.
+            av = T.auto_variant("copy", 4 * k)
+            row["copy_auto"] = {"variant": av,
+                                "gbs": row["copy_gbs"] if av == "shfl" else row["lds_gbs"]["copy"]}
+            row["copy_auto"]["frac_peak"] = round(row["copy_auto"]["gbs"] / HBM_PEAK_GBS, 3)
         row["copy_frac_peak"] = round(row["copy_gbs"] / HBM_PEAK_GBS, 3)
         sweep.append(row)
     # Structs beyond the sweep: 96..1024 bytes (6..64 16-byte words; 96, 160,

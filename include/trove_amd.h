@@ -53,6 +53,17 @@ extern "C" {
 #define TROVE_VARIANT_LDS 4    /* LDS-staged transposition (trove/lds.h): dwordx4 tile traffic, per-lane LDS
                                   reads; struct sizes that are multiples of 4 up to 512 bytes */
 
+/* Operations for trove_auto_variant. */
+#define TROVE_OP_COPY 0
+#define TROVE_OP_GATHER 1
+#define TROVE_OP_SCATTER 2
+
+/* The variant TROVE_VARIANT_AUTO resolves to for trove_copy (tile 64),
+ * trove_gather or trove_scatter at this struct size; TROVE_ERR_SIZE for a
+ * size the library does not compile, TROVE_ERR_INVALID for an unknown op.
+ * No reference counterpart (the reference has no host API). */
+int trove_auto_variant(int op, int struct_bytes);
+
 /* Message for a return code. */
 const char* trove_strerror(int code);
 /* hipError_t of the last TROVE_ERR_HIP on the calling thread. */

@@ -50,6 +50,16 @@ def lds_supported(struct_bytes: int) -> bool:
     return struct_bytes % 4 == 0 and 4 <= struct_bytes <= LDS_MAX_STRUCT_BYTES
 
 
+OPS = {"copy": 0, "gather": 1, "scatter": 2}
+
+
+def auto_variant(op: str, struct_bytes: int) -> str:
+    """Name of the variant AUTO resolves to for copy / gather / scatter at this size."""
+    v = lib().trove_auto_variant(OPS[op], struct_bytes)
+    _check(v if v < 0 else OK, "trove_auto_variant")
+    return {c: name for name, c in VARIANTS.items()}[v]
+
+
 def word_bytes(struct_bytes: int) -> int:
     """Bytes of the word a struct moves in: the widest of 16, 8, 4 dividing it, else 1."""
     for w in (16, 8, 4):
@@ -87,6 +97,7 @@ _SIGNATURES = {
     "trove_version": ([], _int),
     "trove_struct_size_supported": ([_int], _int),
     "trove_struct_word_bytes": ([_int], _int),
+    "trove_auto_variant": ([_int, _int], _int),
     "trove_copy": ([_vp, _vp, _i64, _int, _int, _int, _vp], _int),
     "trove_gather": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
     "trove_scatter": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),

@@ -133,6 +133,15 @@ int trove_struct_size_supported(int struct_bytes) { return find_size(struct_byte
 
 int trove_struct_word_bytes(int struct_bytes) { return struct_bytes > 0 ? word_bytes_of(struct_bytes) : 0; }
 
+int trove_auto_variant(int op, int struct_bytes) {
+    if (op < TROVE_OP_COPY || op > TROVE_OP_SCATTER) return TROVE_ERR_INVALID;
+    if (!find_size(struct_bytes)) return TROVE_ERR_SIZE;
+    if (op == TROVE_OP_COPY) return auto_copy_variant(struct_bytes);
+    // The tile path covers every size that is a whole number of dwords.
+    const int v = op == TROVE_OP_GATHER ? auto_gather_variant(struct_bytes) : auto_scatter_variant(struct_bytes);
+    return v == TROVE_VARIANT_TILE && struct_bytes % 4 != 0 ? TROVE_VARIANT_DIRECT : v;
+}
+
 int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int variant, int tile, void* stream) {
     const size_entry* e = find_size(struct_bytes);
     if (!e) return TROVE_ERR_SIZE;
