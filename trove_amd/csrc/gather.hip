@@ -15,13 +15,29 @@
 
 namespace trove_amd {
 
+// Launch shape of the standalone tile gather per struct size: 64-struct
+// groups in flight per wavefront and work-group size. From interleaved
+// timings of random-permutation gathers on MI355X (2^26 rows up to 64 bytes,
+// 2^24 above), consistent over two runs: four groups at 60 and 128 bytes
+// (+3% and +4.5% over two), four groups in 64-thread work-groups at 64
+// bytes (+2.8%), 64-thread work-groups at 96 bytes (+1.5%); elsewhere
+// gather_groups_in_flight in 256-thread work-groups. (The persistent batch
+// kernel keeps gather_groups_in_flight: its registers are the maximum over
+// every size's branch.)
+template <typename T>
+struct gather_tile_shape {
+    static constexpr int unroll =
+        (sizeof(T) == 60 || sizeof(T) == 64 || sizeof(T) == 128) ? 4 : gather_groups_in_flight<T>();
+    static constexpr int block = (sizeof(T) == 64 || sizeof(T) == 96) ? 64 : kBlock;
+};
+
 // Each wavefront handles UNROLL 64-struct groups (one-shot grid; a capped
 // grid strides).
-template <typename T, int UNROLL>
-__global__ __launch_bounds__(kBlock) void k_gather_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
-                                                        T* __restrict__ dst, int64_t n) {
+template <typename T, int UNROLL, int B>
+__global__ __launch_bounds__(B) void k_gather_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                   T* __restrict__ dst, int64_t n) {
     const int64_t groups = (n + 63) >> 6;
-    for (int64_t g0 = wave_index() * UNROLL; g0 < groups; g0 += wave_count() * UNROLL)
+    for (int64_t g0 = wave_index<B>() * UNROLL; g0 < groups; g0 += wave_count<B>() * UNROLL)
         gather_range_tile<T, UNROLL>(src, idx, dst, n, g0, g0 + UNROLL < groups ? g0 + UNROLL : groups);
 }
 
@@ -179,8 +195,8 @@ int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int
     const T* src = static_cast<const T*>(src_);
     T* dst = static_cast<T*>(dst_);
     constexpr bool tiled = tile_path<T>();
-    constexpr int U = gather_groups_in_flight<T>();
-    const int64_t units = ((n + 63) / 64 + U - 1) / U;
+    typedef gather_tile_shape<T> G;
+    const int64_t units = ((n + 63) / 64 + G::unroll - 1) / G::unroll;
     if (variant == TROVE_VARIANT_AUTO) variant = auto_gather_variant(B);
     if (variant == TROVE_VARIANT_TILE && !tiled) variant = TROVE_VARIANT_DIRECT;
     if (variant == TROVE_VARIANT_LDS) {
@@ -204,8 +220,9 @@ int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int
     }
     if (variant == TROVE_VARIANT_TILE) {
         if constexpr (tiled) {
-            hipLaunchKernelGGL((k_gather_tile<T, U>), dim3((unsigned)grid_for(units > 0 ? units : 1)), dim3(kBlock),
-                               0, s, src, idx, dst, n);
+            hipLaunchKernelGGL((k_gather_tile<T, G::unroll, G::block>),
+                               dim3((unsigned)grid_for(units > 0 ? units : 1, G::block / 64)), dim3(G::block), 0, s,
+                               src, idx, dst, n);
             return launch_status();
         }
     }
