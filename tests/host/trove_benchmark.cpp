@@ -9,8 +9,9 @@
 //     <name>, <k>, <GB/s>, Results passed | INCORRECT
 //
 // GB/s follows the reference's accounting: store S, load S + 4, scatter and
-// gather 2 S + 4 bytes per struct and launch. The tile-path gather and
-// scatter (no reference counterpart) are printed after the reference's lines.
+// gather 2 S + 4 bytes per struct and launch. The tile-path and LDS-staged
+// gather and scatter (no reference counterpart) are printed after the
+// reference's lines.
 // Everything goes through the C ABI (include/trove_amd.h).
 //
 // Usage: trove_benchmark [--n N] [--lower L] [--upper U] [--iterations I] [--seed S]
@@ -202,5 +203,7 @@ int main(int argc, char** argv) {
     for (int k = o.lower; k <= o.upper; ++k) random_access(o, perm, "Direct Gather", k, true, TROVE_VARIANT_DIRECT);
     for (int k = o.lower; k <= o.upper; ++k) random_access(o, perm, "Tile Scatter", k, false, TROVE_VARIANT_TILE);
     for (int k = o.lower; k <= o.upper; ++k) random_access(o, perm, "Tile Gather", k, true, TROVE_VARIANT_TILE);
+    for (int k = o.lower; k <= o.upper; ++k) random_access(o, perm, "LDS Scatter", k, false, TROVE_VARIANT_LDS);
+    for (int k = o.lower; k <= o.upper; ++k) random_access(o, perm, "LDS Gather", k, true, TROVE_VARIANT_LDS);
     return all_passed ? 0 : 1;
 }

@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 
 HOST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "host")
 NAMES = ["Contiguous SHFL Store", "Contiguous Direct Store", "Contiguous SHFL Load", "Contiguous Direct Load",
-         "SHFL Scatter", "Direct Scatter", "SHFL Gather", "Direct Gather", "Tile Scatter", "Tile Gather"]
+         "SHFL Scatter", "Direct Scatter", "SHFL Gather", "Direct Gather", "Tile Scatter", "Tile Gather",
+         "LDS Scatter", "LDS Gather"]
 
 
 def _run(args, timeout=240):
