@@ -33,8 +33,10 @@ __global__ __launch_bounds__(64) void k_block_copy(const uint32_t* __restrict__ 
     constexpr int U = block_shape<S>::unroll;
     const int64_t unit = S >= 5 ? unit_for_xcd(blockIdx.x, (int64_t)gridDim.x) : (int64_t)blockIdx.x;
     // Every group of the unit whole, undisturbed and converged: all loads
-    // before all stores, so that U loads are in flight per wavefront.
-    bool whole = U > 1 && ((unit * U + U) << 6) <= l;
+    // before all stores, so that U loads are in flight per wavefront, and no
+    // per-lane bounds branch around the warp-contiguous accesses (one-group
+    // units included).
+    bool whole = ((unit * U + U) << 6) <= l;
 #pragma unroll
     for (int u = 0; u < U; ++u) whole = whole && !(divergent && divergent_group(seed, unit * U + u));
     if (whole && trove::warp_converged()) {
