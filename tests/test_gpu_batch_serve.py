@@ -64,6 +64,30 @@ def test_batch_modes(trove, cuda, mode):
     b.close()
 
 
+def test_batch_tiny_jobs_every_variant(trove, cuda):
+    # Jobs shorter than one 64-struct group or a few structs past it: only
+    # the per-lane tails run, for gathers and copies of every variant.
+    rng = np.random.default_rng(5)
+    jobs = []
+    for n in (1, 17, 63, 64, 65, 130):
+        for s in (4, 12, 32, 60):
+            for variant in ("tile", "shfl", "lds", "direct"):
+                src = rng.integers(0, 256, size=max(n, 70) * s, dtype=np.uint8)
+                idx = rng.integers(0, max(n, 70), size=n, dtype=np.int32)
+                jobs.append(dict(s=s, n=n, src=src, idx=idx, want=oracle.gather(src, idx, s), variant=variant,
+                                 chunk=1))
+                jobs.append(dict(s=s, n=n, src=src, idx=None, want=oracle.copy(src, n, s), variant=variant,
+                                 chunk=4))
+    b = _device_jobs(jobs)
+    for mode in ("persistent", "looped"):
+        for J in jobs:
+            J["d_dst"].zero_()
+        torch.cuda.synchronize()
+        b.run(mode)
+        _check(jobs, mode)
+    b.close()
+
+
 def test_batch_graph_replays(trove, cuda):
     jobs = _jobs(64, 11)
     b = _device_jobs(jobs)
