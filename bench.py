@@ -405,6 +405,7 @@ def main():
     naive = {}
     if not args.no_naive:
         nclock = KernelClock(torch)
+        sums = torch.empty(n, dtype=torch.int32, device=dev)  # the load benchmark's per-struct sums
         for k in ks:
             s = 4 * k
             for _ in range(3):
@@ -420,6 +421,11 @@ def main():
                 nclock.run(f"scatter_direct_k{k}", (2 * s + 4) * n,
                            lambda: T.scatter(src, idx, dst, n, s, variant="direct"))
                 nclock.run(f"scatter_lds_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="lds"))
+                # The reference benchmark's store-only and load-only kernels
+                # (tests/benchmark.cu:52-88; bytes S and S + 4 per struct).
+                for v in ("shfl", "direct"):
+                    nclock.run(f"store_{v}_k{k}", s * n, lambda v=v: T.store_counting(dst, n, s, variant=v))
+                    nclock.run(f"load_{v}_k{k}", (s + 4) * n, lambda v=v: T.load_sum(src, sums, n, s, variant=v))
         for _ in range(3):
             nclock.run("memcpy_ceiling", 2 * n * smax, lambda: T.memcpy_ceiling(src, dst, n * smax))
         # Block interface: 2^28 32-bit elements, s per lane, converged and
@@ -446,6 +452,7 @@ def main():
             del line_idx, line_out
         torch.cuda.synchronize()
         nclock.collect()
+        del sums
         for name, (launches, ms, nbytes) in nclock.totals.items():
             avg_ms = ms / launches
             naive[name] = {"avg_ms": round(avg_ms, 4), "gbs": round(nbytes / launches / (avg_ms * 1e-3) / 1e9, 1)}
@@ -470,6 +477,8 @@ def main():
                         "scatter_naive_gbs": naive[f"scatter_direct_k{k}"]["gbs"]})
             row["lds_gbs"] = {"copy": naive[f"copy_lds_k{k}"]["gbs"], "gather": naive[f"gather_lds_k{k}"]["gbs"],
                               "scatter": naive[f"scatter_lds_k{k}"]["gbs"]}
+            row["store_gbs"] = {"shfl": naive[f"store_shfl_k{k}"]["gbs"], "naive": naive[f"store_direct_k{k}"]["gbs"]}
+            row["load_gbs"] = {"shfl": naive[f"load_shfl_k{k}"]["gbs"], "naive": naive[f"load_direct_k{k}"]["gbs"]}
             # The path AUTO takes at this size (LDS-staged or register
             # transposition) and its rate from that path's own leg above.
             av = T.auto_variant("copy", 4 * k)
