@@ -41,7 +41,8 @@ def parse_args():
     p.add_argument("--n-log2", type=int, default=26, help="structs per struct size (log2)")
     p.add_argument("--k", type=str, default="1-16", help="struct sizes in 32-bit words, e.g. 1-16 or 5,16")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-n-log2", type=int, default=23, help="structs per size in the CPU sample (log2)")
+    p.add_argument("--cpu-n-log2", type=int, default=26,
+                   help="structs per size in the CPU sample (log2; 26 = the sweep's N, about 10-15 s of CPU work)")
     p.add_argument("--no-naive", action="store_true", help="skip the naive-kernel comparison")
     p.add_argument("--seed", type=int, default=20260101)
     p.add_argument("--config", choices=("all", "sweep", "batched", "served"), default="all",
