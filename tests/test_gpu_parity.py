@@ -109,6 +109,27 @@ def test_scatter_all_variants(trove, cuda, s):
             assert np.array_equal(host(d_dst), want), (s, name, variant)
 
 
+@pytest.mark.parametrize("s", [4, 8, 12, 16, 20, 60, 64, 96, 256])
+def test_gather_scatter_fewer_than_a_group(trove, cuda, s):
+    # n below, at and just past one 64-struct group: every variant's partial
+    # group handling, against the oracle.
+    n_src = 200
+    src = rand_bytes(n_src * s, 60 + s)
+    d_src = dev(src)
+    variants = ("shfl", "tile", "direct") + (("lds",) if T.lds_supported(s) else ())
+    for n in (1, 5, 63, 64, 65, 129):
+        idx = np.random.default_rng(n + s).integers(0, n_src, size=n, dtype=np.int32)
+        perm = np.random.default_rng(n + s + 1).permutation(n).astype(np.int32)
+        init = rand_bytes(n * s, 61 + s)
+        for variant in variants + ("auto",):
+            d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+            T.gather(d_src, dev(idx), d_dst, n, s, variant=variant)
+            assert np.array_equal(host(d_dst), oracle.gather(src, idx, s)), (s, n, variant, "gather")
+            d_dst = dev(init.copy())
+            T.scatter(d_src, dev(perm), d_dst, n, s, variant=variant)
+            assert np.array_equal(host(d_dst), oracle.scatter(src[:n * s], perm, init, s)), (s, n, variant, "scatter")
+
+
 def test_auto_variant_every_size(trove, cuda):
     # AUTO picks a path per struct size (LDS-staged or not); whatever it picks
     # must match the oracle, and a sub-wavefront tile with AUTO must work.
