@@ -11,6 +11,13 @@
 // (lds_stage_bytes<T>() = 64 * sizeof(T) bytes of LDS per wavefront) and two
 // LDS round trips per direction.
 //
+// Structs of 16-byte words are stored swizzled: the m words of struct l are
+// rotated by rot(l) = (l mod 16) / (16 / gcd(m, 16)) slots, so that the 16
+// lanes of one LDS pass read or write 16 different 16-byte bank groups of
+// gfx950's 64 four-byte banks (unswizzled, the lanes of a pass all hit the
+// same group at 128- and 256-byte strides). The stage's contents are
+// therefore only meaningful through these functions.
+//
 // The reference's transposition is register-only (transpose.h); these
 // functions have no counterpart there. Preconditions are those of
 // load_warp_contiguous<64>: all 64 lanes active, lane l passes tile base + l,
@@ -18,6 +25,7 @@
 // 16-byte aligned LDS that no other wavefront uses meanwhile.
 #pragma once
 #include <trove/aos.h>
+#include <trove/static_gcd.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -55,6 +63,35 @@ __device__ __forceinline__ const __attribute__((address_space(3))) V* lds_view(c
     return (const __attribute__((address_space(3))) V*)p;
 }
 
+// LDS slot of the tile's U-words. Identity for 4- and 8-byte words; for
+// 16-byte words (W = 4, m > 1) word j of struct l sits at l * m +
+// (j + rot(l)) mod m, rot(l) = (l mod 16) >> log2(16 / gcd(m, 16)) < m.
+template <typename T>
+struct lds_swizzle {
+    typedef word_layout<T> L;
+    static constexpr bool on = L::W == 4 && L::m > 1;
+    static constexpr int g = on ? static_gcd<L::m, 16>::value : 1;
+    static constexpr int shift = g >= 16 ? 0 : g >= 8 ? 1 : g >= 4 ? 2 : g >= 2 ? 3 : 4;
+    static __device__ __forceinline__ int rot(int l) { return (l & 15) >> shift; }
+    // Slot of word j (0 <= j < m) of struct l.
+    static __device__ __forceinline__ int of(int l, int j) {
+        if constexpr (!on) {
+            return l * L::m + j;
+        } else {
+            const int w = j + rot(l);
+            return l * L::m + (w < L::m ? w : w - L::m);
+        }
+    }
+    // Slot of tile U-word u (struct u / m, word u % m).
+    static __device__ __forceinline__ int of(int u) {
+        if constexpr (!on) {
+            return u;
+        } else {
+            return of(u / L::m, u % L::m);
+        }
+    }
+};
+
 // A tile's bytes as CW-dword chunks in registers: chunk c = lane + 64 r
 // (r < rounds) is tile dwords [CW c, CW c + CW). The last round is partial
 // when 64 * k / CW is not a multiple of 64.
@@ -90,13 +127,27 @@ __device__ __forceinline__ void lds_global_store(T* tile_base, int lane, const l
         if (C::has(lane, r)) store_word<Policy>(p + lane + WARP_SIZE * r, t.c[r]);
 }
 
+// Stage position of chunk c in units of the chunk: a 4-dword chunk of a
+// 16-byte-word tile is one U-word, a dword chunk a quarter of one.
+template <typename S, int CW>
+__device__ __forceinline__ int lds_chunk_pos(int c) {
+    if constexpr (!S::on) {
+        return c;
+    } else if constexpr (CW == 4) {
+        return S::of(c);
+    } else {
+        static_assert(CW == 1, "16-byte-word tiles move in 4- or 1-dword chunks");
+        return 4 * S::of(c >> 2) + (c & 3);
+    }
+}
+
 template <int CW, typename T>
 __device__ __forceinline__ void lds_put_chunks(uint32_t* stage, int lane, const lds_chunks<T, CW>& t) {
     typedef lds_chunks<T, CW> C;
     auto s = lds_view<typename C::vec>(stage);
 #pragma unroll
     for (int r = 0; r < C::rounds; ++r)
-        if (C::has(lane, r)) s[lane + WARP_SIZE * r] = t.c[r];
+        if (C::has(lane, r)) s[lds_chunk_pos<lds_swizzle<T>, CW>(lane + WARP_SIZE * r)] = t.c[r];
 }
 
 template <int CW, typename T>
@@ -105,7 +156,7 @@ __device__ __forceinline__ void lds_get_chunks(const uint32_t* stage, int lane, 
     auto s = lds_view<typename C::vec>(stage);
 #pragma unroll
     for (int r = 0; r < C::rounds; ++r)
-        if (C::has(lane, r)) t.c[r] = s[lane + WARP_SIZE * r];
+        if (C::has(lane, r)) t.c[r] = s[lds_chunk_pos<lds_swizzle<T>, CW>(lane + WARP_SIZE * r)];
 }
 
 // The lane's own struct in the stage, moved as the struct's words.
@@ -114,10 +165,10 @@ __device__ __forceinline__ T lds_get_struct(const uint32_t* stage, int lane) {
     typedef word_layout<T> L;
     typedef dword_vec<L::W> V;
     static_assert(sizeof(V) == sizeof(typename L::U), "word size");
-    auto s = lds_view<V>(stage + lane * L::m * L::W);
+    auto s = lds_view<V>(stage);
     V w[L::m];
 #pragma unroll
-    for (int j = 0; j < L::m; ++j) w[j] = s[j];
+    for (int j = 0; j < L::m; ++j) w[j] = s[lds_swizzle<T>::of(lane, j)];
     T out;
     words_to_struct(w, out);
     return out;
@@ -129,9 +180,9 @@ __device__ __forceinline__ void lds_put_struct(uint32_t* stage, int lane, const 
     typedef dword_vec<L::W> V;
     V w[L::m];
     struct_to_words(v, w);
-    auto s = lds_view<V>(stage + lane * L::m * L::W);
+    auto s = lds_view<V>(stage);
 #pragma unroll
-    for (int j = 0; j < L::m; ++j) s[j] = w[j];
+    for (int j = 0; j < L::m; ++j) s[lds_swizzle<T>::of(lane, j)] = w[j];
 }
 
 // Column-side words (word j of lane l = tile word l + 64 j, as the indexed
@@ -146,7 +197,7 @@ __device__ __forceinline__ void lds_put_columns(uint32_t* stage, int lane,
     for (int j = 0; j < L::m; ++j) {
         V w;
         __builtin_memcpy(&w, &v[j * L::W], sizeof(V));
-        s[lane + WARP_SIZE * j] = w;
+        s[lds_swizzle<T>::of(lane + WARP_SIZE * j)] = w;
     }
 }
 
@@ -158,7 +209,7 @@ __device__ __forceinline__ void lds_get_columns(const uint32_t* stage, int lane,
     auto s = lds_view<V>(stage);
 #pragma unroll
     for (int j = 0; j < L::m; ++j) {
-        const V w = s[lane + WARP_SIZE * j];
+        const V w = s[lds_swizzle<T>::of(lane + WARP_SIZE * j)];
         __builtin_memcpy(&v[j * L::W], &w, sizeof(V));
     }
 }

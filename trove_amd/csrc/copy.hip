@@ -87,6 +87,10 @@ __global__ __launch_bounds__(64) void k_copy_lds(const T* __restrict__ src, T* _
     namespace d = trove::detail;
     constexpr int k = (int)(sizeof(T) / 4);
     typedef unit_words<UNROLL * k> Unit;  // 64 units = the wavefront's UNROLL groups
+    // The chunks take Unit's stage layout and the structs T's: they agree
+    // when the unit is one group, or when neither is swizzled (multi-group
+    // units only occur for 4- and 8-byte structs).
+    static_assert(UNROLL == 1 || (!d::lds_swizzle<T>::on && !d::lds_swizzle<Unit>::on), "stage layouts differ");
     const int lane = trove::lane_id();
     const int64_t groups = n >> 6;
     const int64_t g0 = (int64_t)blockIdx.x * UNROLL;
