@@ -26,9 +26,9 @@ constexpr int kMaxLdsStructBytes = 512;
 // (a full dwordx4 per lane instead of narrower per-lane words) and the
 // 16-byte-word structs of 32 and 96-160 bytes (no two-rotation schedule);
 // gathers: 16, 48, 64 and 192 bytes; scatters: 8 and 16 bytes. 4-byte
-// scatters keep the naive kernel (the streaming one-word path runs at half
-// its rate). Everything else takes the transposition (copy) or the tile path
-// (gather, scatter).
+// scatters keep the naive kernel (0.93 TB/s; the streaming one-word tile
+// scatter reaches 0.84). Everything else takes the transposition (copy) or
+// the tile path (gather, scatter).
 constexpr int auto_copy_variant(int b) {
     return (b == 4 || b == 8 || b == 12 || b == 16 || b == 32 || b == 96 || b == 128 || b == 160)
                ? TROVE_VARIANT_LDS
