@@ -72,9 +72,9 @@ struct lds_swizzle {
     static constexpr bool on = L::W == 4 && L::m > 1;
     static constexpr int g = on ? static_gcd<L::m, 16>::value : 1;
     static constexpr int shift = g >= 16 ? 0 : g >= 8 ? 1 : g >= 4 ? 2 : g >= 2 ? 3 : 4;
-    static __device__ __forceinline__ int rot(int l) { return (l & 15) >> shift; }
+    static __host__ __device__ __forceinline__ int rot(int l) { return (l & 15) >> shift; }
     // Slot of word j (0 <= j < m) of struct l.
-    static __device__ __forceinline__ int of(int l, int j) {
+    static __host__ __device__ __forceinline__ int of(int l, int j) {
         if constexpr (!on) {
             return l * L::m + j;
         } else {
@@ -83,7 +83,7 @@ struct lds_swizzle {
         }
     }
     // Slot of tile U-word u (struct u / m, word u % m).
-    static __device__ __forceinline__ int of(int u) {
+    static __host__ __device__ __forceinline__ int of(int u) {
         if constexpr (!on) {
             return u;
         } else {

@@ -7,11 +7,13 @@
 // (lane c, slot d = c*m + d), and the C2R sequence must invert it, with the
 // C2R pushes landing on distinct (lane, slot) pairs. The dynamic path's word
 // walk (trove/aos.h: word j of rank id belongs to struct (id + j*n) / m at
-// word (id + j*n) % m) must equal that integer division for every rank.
+// word (id + j*n) % m) must equal that integer division for every rank, and
+// the LDS stage swizzle must be a conflict-free permutation (swizzle_checks).
 // Exits non-zero on the first failure.
 #include <trove/aos.h>
 #include <trove/array.h>
 #include <trove/detail/schedule.h>
+#include <trove/lds.h>
 #include <trove/rotate.h>
 #include <trove/utility.h>
 
@@ -127,12 +129,44 @@ static int rotate_checks() {
     return fails;
 }
 
+// LDS stage swizzle (trove/lds.h) for structs of m 16-byte words: the map
+// (struct l, word j) -> slot is a permutation of [0, 64 m) that keeps every
+// struct in its own m slots (the word-index form agrees), and for every word
+// j the 16 structs of each LDS pass land in 16 different 16-byte bank groups
+// (slot mod 16), where the unswizzled layout gives 16 / gcd(m, 16) of them.
+template <int m>
+static int swizzle_checks() {
+    typedef trove::array<uint32_t, 4 * m> T;
+    using S = trove::detail::lds_swizzle<T>;
+    static_assert(S::on, "16-byte words");
+    int f = 0;
+    std::vector<int> seen(64 * m, 0);
+    for (int l = 0; l < 64; ++l)
+        for (int j = 0; j < m; ++j) {
+            const int s = S::of(l, j);
+            if (s < l * m || s >= l * m + m || S::of(l * m + j) != s) ++f;
+            if (s >= 0 && s < 64 * m) seen[s]++;
+        }
+    for (int x : seen) f += x != 1;
+    for (int j = 0; j < m; ++j)
+        for (int p = 0; p < 4; ++p) {
+            int groups = 0;
+            for (int l = 16 * p; l < 16 * p + 16; ++l) groups |= 1 << (S::of(l, j) & 15);
+            f += groups != 0xFFFF;
+        }
+    if (f) std::printf("lds swizzle m=%d: %d failures\n", m, f);
+    return f;
+}
+
 int main() {
     int f = all_m<2, 1>() + all_m<4, 1>() + all_m<8, 1>() + all_m<16, 1>() + all_m<32, 1>() + all_m<64, 1>();
     f += rotate_checks<2>() + rotate_checks<3>() + rotate_checks<5>() + rotate_checks<7>() + rotate_checks<12>() +
          rotate_checks<15>() + rotate_checks<16>();
     f += walk_checks<2, 1>() + walk_checks<4, 1>() + walk_checks<8, 1>() + walk_checks<16, 1>() +
          walk_checks<32, 1>() + walk_checks<64, 1>();
-    std::printf("schedule/rotate/word-walk failures: %d\n", f);
+    f += swizzle_checks<2>() + swizzle_checks<3>() + swizzle_checks<4>() + swizzle_checks<6>() +
+         swizzle_checks<8>() + swizzle_checks<10>() + swizzle_checks<12>() + swizzle_checks<14>() +
+         swizzle_checks<16>() + swizzle_checks<24>() + swizzle_checks<32>();
+    std::printf("schedule/rotate/word-walk/swizzle failures: %d\n", f);
     return f == 0 ? 0 : 1;
 }
