@@ -230,7 +230,12 @@ static int launch_step(trove_serve* s) {
         d.n = cnt;
         d.bytes = s->row_bytes;
         d.variant = s->variant;
-        d.chunk_groups = s->variant == TROVE_VARIANT_SHFL ? 1 : 8;
+        // Work items of two 64-row groups: a step lasts a few hundred
+        // microseconds, and small items shorten its tail. Measured on MI355X
+        // (2^24 x 32-B table, Zipf requests): against eight groups per item,
+        // saturation +3% and p50 / p99 at 80-95% load 20-30% lower; one group
+        // per item costs 7% of saturation throughput.
+        d.chunk_groups = s->variant == TROVE_VARIANT_SHFL ? 1 : 2;
         d.pad = 0;
         d.first_item = items;
         items += ((cnt + 63) / 64 + d.chunk_groups - 1) / d.chunk_groups;
