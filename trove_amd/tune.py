@@ -18,7 +18,7 @@ import trove_amd as T
 SIZES = tuple(range(4, 65, 4))
 PATTERNS = ("identity", "random", "runs")
 LENGTHS = (1 << 10, 1 << 14, 1 << 18, 1 << 22)
-CHUNKS = (1, 4, 16, 64)
+CHUNKS = (1, 2, 4, 16, 64)
 
 
 def bucket(n: int, lengths=LENGTHS) -> int:
