@@ -49,6 +49,7 @@ def parse_args():
                    help="all: the sweep line with batched and served summaries attached")
     p.add_argument("--jobs", type=int, default=1024, help="batched: number of gather jobs")
     p.add_argument("--tune-table", type=str, default=None, help="batched: tuner table to load instead of tuning")
+    p.add_argument("--serve-slots", type=int, default=1024, help="served: request slots of the server")
     p.add_argument("--requests", type=int, default=8192, help="served: number of requests")
     return p.parse_args()
 
@@ -276,10 +277,15 @@ def bench_served(args, torch, T):
     budget = 4 << 30
     total_bytes = 68 * int(sizes.sum())
 
+    # Slices of 2^18 indices, the workload's largest request, so that every
+    # request finishes in one step (measured: 2^15-2^17 give 2-12% lower
+    # saturation and up to 1.5x the p99 at 95% load).
+    slice_indices = 1 << 18
+
     def run(arrivals):
         k = len(arrivals)
-        srv = T.Server(table, n_rows, rb, budget_bytes=budget, max_slots=1024, slice_indices=1 << 16,
-                       variant="tile")
+        srv = T.Server(table, n_rows, rb, budget_bytes=budget, max_slots=args.serve_slots,
+                       slice_indices=slice_indices, variant="tile")
         t_start = time.perf_counter()
         lat = srv.replay(arrivals, ptrs[:k], sizes[:k], on_device=True, release_on_done=True)
         wall = time.perf_counter() - t_start
@@ -292,6 +298,7 @@ def bench_served(args, torch, T):
     sat_rate = R / wall
     out = {
         "workload": "table 2^24 x 32 B; requests log-uniform 2^8..2^18 indices, Zipf(1.1) rows; 4 GiB budget",
+        "server": {"slots": args.serve_slots, "slice_indices": slice_indices, "variant": "tile"},
         "requests": R, "indices": int(sizes.sum()),
         "saturation": {"req_per_s": round(sat_rate, 1), "gbs": round(total_bytes / wall / 1e9, 1),
                        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
