@@ -182,9 +182,40 @@ __global__ __launch_bounds__(64 * WAVES) void k_lds_roundtrip(const unsigned cha
     trove::store_warp_contiguous_lds(v, d + lane, my);
 }
 
+// A user struct of mixed field types (8-byte aligned, 24 bytes = three
+// 8-byte words): a coalesced_ptr gather written as ref = ref, the register
+// transposition and the LDS-staged pair, one wavefront.
+struct particle {
+    float x, y, z;
+    int id;
+    double mass;
+};
+static_assert(sizeof(particle) == 24 && alignof(particle) == 8, "particle layout");
+
+__global__ __launch_bounds__(64) void k_user_struct(const particle* src, const int* perm, particle* gathered,
+                                                    particle* copied, particle* lds_copied) {
+    __shared__ __align__(16) uint32_t stage[64 * sizeof(particle) / 4];
+    const int lane = threadIdx.x;
+    trove::coalesced_ptr<particle> s(const_cast<particle*>(src));
+    trove::coalesced_ptr<particle> g(gathered);
+    g[lane] = s[perm[lane]];
+    const particle v = trove::load_warp_contiguous(src + lane);
+    trove::store_warp_contiguous(v, copied + lane);
+    const particle w = trove::load_warp_contiguous_lds(src + lane, stage);
+    trove::store_warp_contiguous_lds(w, lds_copied + lane, stage);
+}
+
 }  // namespace
 
 extern "C" {
+
+// 64 particles (24 bytes each) in src; outputs of the same size.
+int devtest_user_struct(const void* src, const int* perm, void* gathered, void* copied, void* lds_copied) {
+    hipLaunchKernelGGL(k_user_struct, dim3(1), dim3(64), 0, 0, (const particle*)src, perm, (particle*)gathered,
+                       (particle*)copied, (particle*)lds_copied);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -2;
+}
+
 
 // Returns 0 and fills r2c_out / c2r_out (64 * m words each), -1 for an
 // unsupported combination, -2 on a launch error.

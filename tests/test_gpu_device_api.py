@@ -43,6 +43,8 @@ def dt(cuda):
     L.devtest_mixed.restype = i
     L.devtest_lds.argtypes = [i, i, i, vp, vp, vp]
     L.devtest_lds.restype = i
+    L.devtest_user_struct.argtypes = [vp, vp, vp, vp, vp]
+    L.devtest_user_struct.restype = i
     for f in (L.devtest_transpose, L.devtest_rotate, L.devtest_converged, L.devtest_shfl, L.devtest_cascade):
         f.restype = i
     return L
@@ -211,6 +213,27 @@ def test_lds_staged_api(dt, s):
         want = init.copy()
         want[off:off + tiles * 64 * s] = body
         assert np.array_equal(d_dst.cpu().numpy(), want), (s, off, "store")
+
+
+def test_user_struct_of_mixed_fields(dt):
+    # struct {float x, y, z; int id; double mass;} (24 bytes, 8-byte words):
+    # coalesced_ptr gather (ref = ref), load/store_warp_contiguous and the
+    # LDS-staged pair move it byte for byte, NaN and denormal floats included.
+    rng = np.random.default_rng(77)
+    src = rng.integers(0, 256, size=64 * 24, dtype=np.uint8)
+    f = src.view(np.uint32).reshape(64, 6)
+    f[::5, 0] = 0x7FC00001  # quiet NaN with payload
+    f[1::7, 1] = 0x00000003  # denormal
+    f[2::9, 2] = 0xFF800000  # -inf
+    perm = rng.permutation(64).astype(np.int32)
+    d_src = torch.from_numpy(src).cuda()
+    outs = [torch.zeros(64 * 24, dtype=torch.uint8, device="cuda") for _ in range(3)]
+    assert dt.devtest_user_struct(d_src.data_ptr(), torch.from_numpy(perm).cuda().data_ptr(),
+                                  *(o.data_ptr() for o in outs)) == 0
+    rows = src.reshape(64, 24)
+    assert np.array_equal(outs[0].cpu().numpy().reshape(64, 24), rows[perm]), "coalesced_ptr gather"
+    assert np.array_equal(outs[1].cpu().numpy(), src), "warp-contiguous copy"
+    assert np.array_equal(outs[2].cpu().numpy(), src), "LDS-staged copy"
 
 
 @pytest.mark.parametrize("s", [12, 20, 32, 48, 96])
