@@ -366,6 +366,41 @@ def test_full_size_large_structs(trove, cuda, s):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("s", [4, 64])
+def test_gather_scatter_rows_near_the_index_limit(trove, cuda, s):
+    # An 8 GiB source: 4-byte rows up to index 2^31 - 2 (the int32 index
+    # limit) and 64-byte rows at byte offsets up to 8 GiB, gathered through
+    # every path and scattered back, against torch indexing.
+    k = s // 4
+    n_src = ((1 << 33) // s) - 1
+    d_src = torch.empty(n_src * k, dtype=torch.int32, device=cuda)
+    T.fill_words_device(d_src, d_src.numel(), 91 + s)
+    rng = np.random.default_rng(s)
+    n = 64 * 70 + 17
+    idx = np.concatenate([n_src - 1 - rng.permutation(n - 5), [0, 1, n_src - 1, n_src // 2, n_src - 2]])
+    idx = idx.astype(np.int32)
+    d_idx = dev(idx)
+    rows = d_src.view(n_src, k)
+    want = rows[d_idx.long()]
+    for variant in ("tile", "shfl", "lds", "direct"):
+        d_dst = torch.zeros(n * k, dtype=torch.int32, device=cuda)
+        T.gather(d_src, d_idx, d_dst, n, s, variant=variant)
+        torch.cuda.synchronize()
+        assert torch.equal(d_dst.view(n, k), want), (s, variant, "gather")
+    # Scatter distinct rows near the top back into a copy of their place.
+    uniq = np.unique(idx).astype(np.int32)
+    m = uniq.size
+    d_u = dev(uniq)
+    payload = torch.arange(m * k, dtype=torch.int32, device=cuda)
+    for variant in ("tile", "shfl", "lds", "direct"):
+        T.scatter(payload, d_u, d_src, m, s, variant=variant)
+        torch.cuda.synchronize()
+        assert torch.equal(rows[d_u.long()], payload.view(m, k)), (s, variant, "scatter")
+        T.fill_words_device(payload, payload.numel(), 7 + len(variant))
+    del d_src, rows, want
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("s", [1, 2, 5, 15])
 def test_full_size_block_interface(trove, cuda, s):
     # The block-interface workload at its full shape: 2^28 32-bit elements,
