@@ -221,6 +221,26 @@ def test_counting_store_and_load_sum(trove, cuda):
                 assert np.array_equal(host(out).view(np.uint32), oracle.load_sum(src, n, k)), (k, n, variant)
 
 
+@pytest.mark.parametrize("k", [1, 3, 4, 5, 8, 10, 13, 16, 24, 40, 64])
+def test_counting_store_and_load_sum_many_groups(trove, cuda, k):
+    # Enough 64-struct groups that the launch spreads work-groups over all
+    # XCDs in XCD-contiguous order under a residency cap, with a partial
+    # last group; sizes up to 256 bytes.
+    s = 4 * k
+    n = (1 << 20) + 37 if k <= 16 else (1 << 18) + 37
+    want_words = oracle.counting_store(n, k)
+    src = np.random.default_rng(100 + k).integers(0, 2**32, size=n * k, dtype=np.uint64).astype(np.uint32)
+    want_sum = oracle.load_sum(src, n, k)
+    d_src = dev(src.view(np.uint8))
+    for variant in ("shfl", "direct", "auto"):
+        d = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+        T.store_counting(d, n, s, variant=variant)
+        assert np.array_equal(host(d).view(np.uint32), want_words), (k, variant)
+        out = torch.zeros(n, dtype=torch.int32, device=cuda)
+        T.load_sum(d_src, out, n, s, variant=variant)
+        assert np.array_equal(host(out).view(np.uint32), want_sum), (k, variant)
+
+
 @pytest.mark.parametrize("s", range(1, 17))
 def test_block_copy(trove, cuda, s):
     e = 64 * s * 53 + 3 * s + (s - 1)

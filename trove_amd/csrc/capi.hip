@@ -42,24 +42,35 @@ int device_cu_count() {
     return cached_cus;
 }
 
-size_t stream_lds_reservation(int64_t bytes_per_block) {
+static int lds_per_cu() {
     static thread_local int cached_dev = -1;
-    static thread_local int lds_per_cu = 0;
+    static thread_local int lds = 0;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 160 * 1024;
     if (dev != cached_dev) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
             v <= 0)
             v = 160 * 1024;
         cached_dev = dev;
-        lds_per_cu = v;
+        lds = v;
     }
+    return lds;
+}
+
+size_t cap_lds_reservation(int blocks_per_cu) {
+    if (blocks_per_cu <= 0) return 0;
+    const int units = lds_per_cu() / kLdsAllocBytes;
+    const int per = units / blocks_per_cu;
+    return per > 0 ? (size_t)per * kLdsAllocBytes : 0;
+}
+
+size_t stream_lds_reservation(int64_t bytes_per_block) {
     if (bytes_per_block <= 0) return 0;
     int64_t groups = kStreamWindowBytes / bytes_per_block;
     if (groups >= 32) return 0;  // a CU holds at most 32 wavefronts anyway
     if (groups < 4) groups = 4;
-    const int64_t per = (int64_t)lds_per_cu / groups;
+    const int64_t per = (int64_t)lds_per_cu() / groups;
     return (size_t)(per / 512 * 512);
 }
 

@@ -216,12 +216,23 @@ __device__ __forceinline__ T counting_struct(int64_t i) {
     return trove::counting_array<T>::impl((uint32_t)(i * k));
 }
 
-template <typename T, int UNROLL, int B>
+// Wavefront w of the one-shot grid (XCD: work-groups in XCD-contiguous order).
+template <int B, bool XCD>
+__device__ __forceinline__ int64_t bench_wave() {
+    if constexpr (XCD) {
+        return unit_for_xcd(blockIdx.x, (int64_t)gridDim.x) * (B / 64) +
+               __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    } else {
+        return wave_index<B>();
+    }
+}
+
+template <typename T, int UNROLL, int B, bool XCD>
 __global__ __launch_bounds__(B) void k_counting_shfl(T* __restrict__ dst, int64_t n) {
     const int lane = trove::lane_id();
     const int64_t groups = n >> 6;
     const int64_t step = wave_count<B>() * UNROLL;
-    int64_t g = wave_index<B>() * UNROLL;
+    int64_t g = bench_wave<B, XCD>() * UNROLL;
     for (; g + UNROLL <= groups; g += step) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -245,13 +256,13 @@ __global__ __launch_bounds__(kBlock) void k_counting_direct(T* __restrict__ dst,
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = counting_struct<T>(i);
 }
 
-template <typename T, int UNROLL, int B>
+template <typename T, int UNROLL, int B, bool XCD>
 __global__ __launch_bounds__(B) void k_loadsum_shfl(const T* __restrict__ src, uint32_t* __restrict__ out,
                                                     int64_t n) {
     const int lane = trove::lane_id();
     const int64_t groups = n >> 6;
     const int64_t step = wave_count<B>() * UNROLL;
-    int64_t g = wave_index<B>() * UNROLL;
+    int64_t g = bench_wave<B, XCD>() * UNROLL;
     for (; g + UNROLL <= groups; g += step) {
         T v[UNROLL];
 #pragma unroll
@@ -277,18 +288,41 @@ __global__ __launch_bounds__(kBlock) void k_loadsum_direct(const T* __restrict__
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) out[i] = trove::sum(src[i]);
 }
 
-// Work-group size of the store-only and load-only kernels. A store-only
-// wavefront of small structs finishes so quickly that dispatching 64-thread
-// work-groups bounds the kernel: measured on MI355X (2^26 structs, streaming
-// policy), 256-thread work-groups store 4-16-byte and 32-byte structs at
-// 5.9-6.6 TB/s against 3.5-5.3 with 64 threads, and load 12- and 16-byte
-// structs at 6.1-6.2 against 4.7-5.9; from 20 bytes up 64 threads are as good
-// or better (the copy's shape).
+// Launch shape of the store-only and load-only kernels per struct size, in
+// 4-byte words: work-group size, XCD-contiguous work-group order, and a cap
+// on resident work-groups per CU (cap_lds_reservation; 0 = none). Chosen on
+// MI355X from sweeps at 2^24 and 2^26 structs (single launches and ten back
+// to back), then checked A/B against the previous shapes with the reference
+// benchmark program at 16,384,000 structs and at 2^26:
+// - A store-only wavefront of small structs finishes so quickly that
+//   dispatching 64-thread work-groups bounds the kernel: 256-thread
+//   work-groups store 12-20-byte structs at 5.7-7.1 TB/s against 3.1-5.9
+//   with 64 threads.
+// - XCD order lifts stores from 12 bytes and loads from 20 bytes by 2-14%;
+//   it costs the 16-byte store 4-8% at 2^24 structs.
+// - Stores of 20 and 32-64 bytes stream best with a cap of 2-11 work-groups
+//   (5-16 wavefronts, about 10-25 KiB of stores per CU): 5.9-7.1 TB/s in
+//   the reference program at its size against 5.3-6.5 uncapped in dispatch
+//   order (0.93-1.3x of it at 2^26). Loads of 44-52 and 60-64 bytes take a
+//   looser cap of 16-21 work-groups.
+// One-word structs of 4, 8 and 16 bytes keep 256-thread work-groups in
+// dispatch order; above 64 bytes only the XCD order applies (no cap measured
+// there).
 template <typename T>
 struct bench_shape {
     static constexpr int words = (int)(sizeof(T) / 4);
-    static constexpr int store_block = (words <= 4 || words == 8) ? 256 : 64;
+    static constexpr int store_block =
+        (words <= 5 || words == 8 || words == 10 || words == 11 || words == 13) ? 256 : 64;
+    static constexpr int store_cap = words == 5                                   ? 4
+                                     : words == 8 || words == 10 || words == 11 || words == 13 ? 2
+                                     : words == 6                                 ? 11
+                                     : words == 9                                 ? 8
+                                     : words == 12 || (words >= 14 && words <= 16) ? 5
+                                                                                   : 0;
+    static constexpr bool store_xcd = words == 3 || words >= 5;
     static constexpr int load_block = (words == 3 || words == 4) ? 256 : 64;
+    static constexpr int load_cap = words >= 11 && words <= 13 ? 21 : words >= 15 && words <= 16 ? 16 : 0;
+    static constexpr bool load_xcd = words >= 5;
 };
 
 template <int B>
@@ -300,10 +334,12 @@ int counting_impl(void* dst_, int64_t n, int variant, hipStream_t s) {
         typedef copy_shape<T> C;
         T* dst = static_cast<T*>(dst_);
         if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL) {
-            constexpr int WB = bench_shape<T>::store_block;
+            typedef bench_shape<T> H;
+            constexpr int WB = H::store_block;
             const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
             const int64_t blocks = grid_for(units, WB / 64);
-            hipLaunchKernelGGL((k_counting_shfl<T, C::unroll, WB>), dim3((unsigned)blocks), dim3(WB), 0, s, dst, n);
+            hipLaunchKernelGGL((k_counting_shfl<T, C::unroll, WB, H::store_xcd>), dim3((unsigned)blocks), dim3(WB),
+                               cap_lds_reservation(H::store_cap), s, dst, n);
             return launch_status();
         }
         if (variant == TROVE_VARIANT_DIRECT) {
@@ -324,11 +360,12 @@ int loadsum_impl(const void* src_, uint32_t* out, int64_t n, int variant, hipStr
         typedef copy_shape<T> C;
         const T* src = static_cast<const T*>(src_);
         if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL) {
-            constexpr int WB = bench_shape<T>::load_block;
+            typedef bench_shape<T> H;
+            constexpr int WB = H::load_block;
             const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
             const int64_t blocks = grid_for(units, WB / 64);
-            hipLaunchKernelGGL((k_loadsum_shfl<T, C::unroll, WB>), dim3((unsigned)blocks), dim3(WB), 0, s, src, out,
-                               n);
+            hipLaunchKernelGGL((k_loadsum_shfl<T, C::unroll, WB, H::load_xcd>), dim3((unsigned)blocks), dim3(WB),
+                               cap_lds_reservation(H::load_cap), s, src, out, n);
             return launch_status();
         }
         if (variant == TROVE_VARIANT_DIRECT) {
