@@ -54,6 +54,49 @@ def test_copy_all_variants(trove, cuda, s):
         assert e.value.code == T.ERR_VARIANT
 
 
+def test_seeded_random_cases(trove, cuda):
+    # 120 cases drawn from one seed: struct size, operation, variant (and tile
+    # for copies), length log-uniform over 1..8000, gather indices with
+    # repeats, scatters of distinct indices into a larger pre-filled
+    # destination; each bit-exact against the oracle. The case is in the
+    # assertion message.
+    rng = np.random.default_rng(20240611)
+    for case in range(120):
+        s = int(rng.choice(SIZES))
+        op = str(rng.choice(["copy", "gather", "scatter"]))
+        n = int(np.exp(rng.uniform(0, np.log(8000))))
+        lds = ("lds",) if T.lds_supported(s) else ()
+        tile = 64
+        if op == "copy":
+            variant = str(rng.choice(("shfl", "direct", "auto") + lds))
+            if variant == "shfl":
+                tile = int(rng.choice(TILES))
+            elif variant == "auto":
+                tile = int(rng.choice((64, 32)))
+        else:
+            variant = str(rng.choice(("shfl", "tile", "direct", "auto") + lds))
+        what = (case, op, s, n, variant, tile)
+        src = rand_bytes(n * s, 5000 + case)
+        if op == "copy":
+            d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+            T.copy(dev(src), d_dst, n, s, variant=variant, tile=tile)
+            assert np.array_equal(host(d_dst), oracle.copy(src, n, s)), what
+        elif op == "gather":
+            n_src = int(rng.integers(1, 3 * n + 2))
+            table = rand_bytes(n_src * s, 6000 + case)
+            idx = rng.integers(0, n_src, size=n, dtype=np.int32)
+            d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+            T.gather(dev(table), dev(idx), d_dst, n, s, variant=variant)
+            assert np.array_equal(host(d_dst), oracle.gather(table, idx, s)), what
+        else:
+            n_dst = n + int(rng.integers(0, 2 * n + 1))
+            idx = rng.permutation(n_dst)[:n].astype(np.int32)
+            init = rand_bytes(n_dst * s, 7000 + case)
+            d_dst = dev(init.copy())
+            T.scatter(dev(src), dev(idx), d_dst, n, s, variant=variant)
+            assert np.array_equal(host(d_dst), oracle.scatter(src, idx, init, s)), what
+
+
 @pytest.mark.parametrize("s", [4, 12, 20, 32, 60, 64, 96, 160])
 def test_copy_partial_wavefront_cascade(trove, cuda, s):
     # Every trailing-group length 1..64 runs the tile cascade once.
