@@ -50,6 +50,8 @@ def parse_args():
     p.add_argument("--jobs", type=int, default=1024, help="batched: number of gather jobs")
     p.add_argument("--tune-table", type=str, default=None, help="batched: tuner table to load instead of tuning")
     p.add_argument("--serve-slots", type=int, default=1024, help="served: request slots of the server")
+    p.add_argument("--serve-variant", choices=("tile", "lds", "shfl", "direct", "auto"), default="tile",
+                   help="served: gather path of the server's steps")
     p.add_argument("--requests", type=int, default=8192, help="served: number of requests")
     return p.parse_args()
 
@@ -285,7 +287,7 @@ def bench_served(args, torch, T):
     def run(arrivals):
         k = len(arrivals)
         srv = T.Server(table, n_rows, rb, budget_bytes=budget, max_slots=args.serve_slots,
-                       slice_indices=slice_indices, variant="tile")
+                       slice_indices=slice_indices, variant=args.serve_variant)
         t_start = time.perf_counter()
         lat = srv.replay(arrivals, ptrs[:k], sizes[:k], on_device=True, release_on_done=True)
         wall = time.perf_counter() - t_start
@@ -298,7 +300,7 @@ def bench_served(args, torch, T):
     sat_rate = R / wall
     out = {
         "workload": "table 2^24 x 32 B; requests log-uniform 2^8..2^18 indices, Zipf(1.1) rows; 4 GiB budget",
-        "server": {"slots": args.serve_slots, "slice_indices": slice_indices, "variant": "tile"},
+        "server": {"slots": args.serve_slots, "slice_indices": slice_indices, "variant": args.serve_variant},
         "requests": R, "indices": int(sizes.sum()),
         "saturation": {"req_per_s": round(sat_rate, 1), "gbs": round(total_bytes / wall / 1e9, 1),
                        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
