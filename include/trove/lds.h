@@ -9,7 +9,9 @@
 // with k = sizeof(T) / 4, as 4-, 8- or 16-byte words like the register path).
 // No cross-lane shuffles and no rotations; the price is the stage
 // (lds_stage_bytes<T>() = 64 * sizeof(T) bytes of LDS per wavefront) and two
-// LDS round trips per direction.
+// LDS round trips per direction. For occupancy, gfx950 allocates a
+// work-group's LDS in 1280-byte units, 128 per CU (counted on MI355X), so a
+// work-group's stages cost their total rounded up to a multiple of 1280.
 //
 // Structs of 16-byte words are stored swizzled: the m words of struct l are
 // rotated by rot(l) = (l mod 16) / (16 / gcd(m, 16)) slots, so that the 16
