@@ -53,6 +53,12 @@ def parse_args():
     p.add_argument("--serve-variant", choices=("tile", "lds", "shfl", "direct", "auto"), default="tile",
                    help="served: gather path of the server's steps")
     p.add_argument("--requests", type=int, default=8192, help="served: number of requests")
+    p.add_argument("--profile-legs", type=str, default=None, metavar="PATH",
+                   help="profiling run: every leg launches once after a one-block marker copy and a "
+                        "synchronisation; the leg order is written to PATH (JSON) for --traffic-from")
+    p.add_argument("--traffic-from", nargs=3, default=None, metavar=("FETCH_DIR", "WRITE_DIR", "LEGS_JSON"),
+                   help="no GPU: build profiles/traffic.json from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                        "output directories of a --profile-legs run, print the table as markdown")
     return p.parse_args()
 
 
@@ -67,10 +73,52 @@ def parse_k(spec: str) -> list[int]:
     return out
 
 
+ROOT = os.path.dirname(os.path.abspath(__file__))
+TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def kernel_sources_sha256() -> str:
+    """Hash of the kernel sources (trove_amd/csrc, include/): the stamp that
+    ties profiles/traffic.json to the kernels it was measured on."""
+    import hashlib
+
+    h = hashlib.sha256()
+    files = []
+    for top in (os.path.join(ROOT, "trove_amd", "csrc"), os.path.join(ROOT, "include")):
+        for d, _, names in os.walk(top):
+            files += [os.path.join(d, f) for f in names if f.endswith((".hip", ".h", ".cpp"))]
+    for f in sorted(files, key=lambda x: os.path.relpath(x, ROOT)):
+        h.update(os.path.relpath(f, ROOT).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+class ProfileLegs:
+    """--profile-legs: the leg sequence of a profiling run. Each leg is one
+    launch preceded by a marker (a one-block trove_memcpy_ceiling of 16
+    bytes, a k_memcpy16 dispatch of grid size 256) and a synchronisation, so
+    that the dispatches of a counter run can be attributed to legs in order."""
+
+    legs = None  # list of [name, algorithmic bytes] when profiling
+    _marker = None
+
+    @classmethod
+    def run(cls, torch, T, name, nbytes, fn):
+        if cls._marker is None:
+            cls._marker = torch.zeros(8, dtype=torch.int32, device="cuda:0")
+        torch.cuda.synchronize()
+        T.memcpy_ceiling(cls._marker[:4], cls._marker[4:], 16)
+        fn()
+        torch.cuda.synchronize()
+        cls.legs.append([name, int(nbytes)])
+
+
 class KernelClock:
     """HIP events around each launch on the launch stream (torch's current).
     The first launch of every name is untimed: HIP loads a kernel's code
-    object lazily at its first launch."""
+    object lazily at its first launch. In a --profile-legs run every launch
+    is a marked leg instead (no timing)."""
 
     def __init__(self, torch, seen=None):
         self.torch = torch
@@ -81,6 +129,11 @@ class KernelClock:
 
     def run(self, name, nbytes, fn):
         t = self.torch
+        if ProfileLegs.legs is not None:
+            import trove_amd as T
+
+            ProfileLegs.run(t, T, name, nbytes, fn)
+            return
         if name not in self.seen:
             self.seen.add(name)
             fn()
@@ -91,6 +144,8 @@ class KernelClock:
         self.pending.append((name, nbytes, s, e))
 
     def collect(self):
+        if ProfileLegs.legs is not None:
+            return
         for name, nbytes, s, e in self.pending:
             rec = self.totals.setdefault(name, [0, 0.0, 0])
             ms = s.elapsed_time(e)
@@ -138,13 +193,6 @@ def cpu_baseline(ks, n_log2, seed):
     }
 
 
-def _threaded(fn, items, workers=16):
-    from concurrent.futures import ThreadPoolExecutor
-
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        return list(ex.map(fn, items))
-
-
 def timed_runs(torch, fn, reps):
     out = []
     for _ in range(reps):
@@ -166,54 +214,31 @@ def bench_batched(args, torch, T):
     import numpy as np
 
     from trove_amd import tune
+    from trove_amd.workloads import BatchedWorkload
 
-    rng = np.random.default_rng(args.seed + 3)
-    nj = args.jobs
-    sizes = 4 * rng.integers(1, 17, size=nj)
-    lengths = T.gen_loguniform(nj, 1 << 10, 1 << 22, args.seed + 4)
-    patterns = [tune.PATTERNS[p] for p in rng.integers(0, 3, size=nj)]
-    align = lambda x: (x + 255) // 256 * 256
-    src_off, idx_off, tot_src, tot_idx = [], [], 0, 0
-    for s, L in zip(sizes, lengths):
-        src_off.append(tot_src)
-        tot_src += align(int(s) * int(L))
-        idx_off.append(tot_idx)
-        tot_idx += align(4 * int(L)) // 4
-    dev = torch.device("cuda:0")
-    src = torch.empty(tot_src // 4, dtype=torch.int32, device=dev)
-    dst_a = torch.empty_like(src)
-    dst_b = torch.empty_like(src)
-    T.fill_words_device(src, src.numel(), args.seed + 5)
-    t0 = time.time()
-    host_idx = np.empty(tot_idx, dtype=np.int32)
-
-    def gen(j):
-        L = int(lengths[j])
-        host_idx[idx_off[j]:idx_off[j] + L] = T.gen_indices(L, L, patterns[j], args.seed * 7919 + j)
-
-    _threaded(gen, range(nj))
-    idx = torch.from_numpy(host_idx).to(dev)
-    log(f"bench batched: {nj} jobs, {int(lengths.sum())} structs, inputs in {time.time() - t0:.1f}s")
+    W = BatchedWorkload(args.jobs, args.seed, torch)
+    nj = W.n_jobs
+    log(f"bench batched: {nj} jobs, {W.structs} structs, inputs in {W.setup_s:.1f}s")
     if args.tune_table and os.path.exists(args.tune_table):
         table = tune.load(args.tune_table)
     else:
         t0 = time.time()
         table = tune.autotune(reps=3, seed=args.seed)
         log(f"bench batched: tuned {len(table['entries'])} shapes in {time.time() - t0:.1f}s")
-    base = src.data_ptr()
-    jobs_a, jobs_b, choice = [], [], {}
-    for j in range(nj):
-        s, L = int(sizes[j]), int(lengths[j])
-        v, c = tune.choose(table, s, patterns[j], L)
+    choice = {}
+
+    def choose(s, pattern, L):
+        v, c = tune.choose(table, s, pattern, L)
         choice[v] = choice.get(v, 0) + 1
-        sp, ip = base + src_off[j], idx.data_ptr() + 4 * idx_off[j]
-        jobs_a.append((sp, ip, dst_a.data_ptr() + src_off[j], L, s, v, c))
-        jobs_b.append((sp, ip, dst_b.data_ptr() + src_off[j], L, s, v, c))
-    ba, bb = T.Batch(jobs_a), T.Batch(jobs_b)
+        return v, c
+
+    ba = T.Batch(W.jobs(choose, W.dst_a))
+    bb = T.Batch(W.jobs(lambda s, p, L: tune.choose(table, s, p, L), W.dst_b))
     items, nbytes = ba.stats()
     ba.run("persistent")
     bb.run("looped")
     torch.cuda.synchronize()
+    dst_a, dst_b = W.dst_a, W.dst_b
     same = all(torch.equal(dst_a[i:i + (1 << 26)], dst_b[i:i + (1 << 26)]) for i in range(0, dst_a.numel(), 1 << 26))
     reps = 5
     t_pers = statistics.median(timed_runs(torch, lambda: ba.run("persistent"), reps))
@@ -225,18 +250,21 @@ def bench_batched(args, torch, T):
 
     checked = 0
     for j in list(range(0, nj, max(1, nj // 8)))[:8]:
-        s, L = int(sizes[j]), int(lengths[j])
-        src_h = src[src_off[j] // 4:(src_off[j] + s * L) // 4].cpu().numpy().view(np.uint8)
-        want = oracle.gather(src_h, host_idx[idx_off[j]:idx_off[j] + L], s)
-        got = dst_a[src_off[j] // 4:(src_off[j] + s * L) // 4].cpu().numpy().view(np.uint8)
+        w0, nw = W.job_words(j)
+        s = int(W.sizes[j])
+        src_h = W.src[w0:w0 + nw].cpu().numpy().view(np.uint8)
+        want = oracle.gather(src_h, W.job_indices(j), s)
+        got = dst_a[w0:w0 + nw].cpu().numpy().view(np.uint8)
         same = same and np.array_equal(got, want)
         checked += 1
     ba.close()
     bb.close()
+    structs = W.structs
+    del W
     return {
         "workload": (f"{nj} gather jobs, struct 4..64 B uniform, length log-uniform 2^10..2^22, pattern uniform "
                      "{identity, random permutation, sorted runs mean 32}"),
-        "jobs": nj, "structs": int(lengths.sum()), "work_items": items, "algorithmic_bytes": nbytes,
+        "jobs": nj, "structs": structs, "work_items": items, "algorithmic_bytes": nbytes,
         "persistent_ms": round(t_pers, 3), "graph_ms": round(t_graph, 3), "looped_ms": round(t_loop, 3),
         "persistent_jobs_per_s": round(nj / (t_pers * 1e-3), 1),
         "graph_jobs_per_s": round(nj / (t_graph * 1e-3), 1),
@@ -258,24 +286,14 @@ def bench_served(args, torch, T):
     budget."""
     import numpy as np
 
-    dev = torch.device("cuda:0")
-    n_rows, rb = 1 << 24, 32
-    table = torch.empty(n_rows * rb // 4, dtype=torch.int32, device=dev)
-    T.fill_words_device(table, table.numel(), args.seed + 11)
-    R = args.requests
-    sizes = T.gen_loguniform(R, 1 << 8, 1 << 18, args.seed + 12)
-    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
-    host_idx = np.empty(int(sizes.sum()), dtype=np.int32)
-    t0 = time.time()
+    from trove_amd.workloads import ServedWorkload
 
-    def gen(i):
-        host_idx[offs[i]:offs[i] + sizes[i]] = T.gen_zipf(int(sizes[i]), n_rows, 1.1, args.seed * 31 + i)
-
-    _threaded(gen, range(R))
-    idx = torch.from_numpy(host_idx).to(dev)
+    W = ServedWorkload(args.requests, args.seed, torch)
+    table, n_rows, rb = W.table, W.n_rows, W.row_bytes
+    R, sizes, offs, host_idx = W.n_requests, W.sizes, W.offs, W.host_idx
     torch.cuda.synchronize()
-    log(f"bench served: {R} requests, {host_idx.size} indices, inputs in {time.time() - t0:.1f}s")
-    ptrs = idx.data_ptr() + 4 * offs
+    log(f"bench served: {R} requests, {host_idx.size} indices, inputs in {W.setup_s:.1f}s")
+    ptrs = W.device_pointers()
     budget = 4 << 30
     total_bytes = 68 * int(sizes.sum())
 
@@ -284,7 +302,7 @@ def bench_served(args, torch, T):
     # saturation and up to 1.5x the p99 at 95% load).
     slice_indices = 1 << 18
 
-    def run(arrivals):
+    def run(arrivals, slice_indices=slice_indices):
         k = len(arrivals)
         srv = T.Server(table, n_rows, rb, budget_bytes=budget, max_slots=args.serve_slots,
                        slice_indices=slice_indices, variant=args.serve_variant)
@@ -306,6 +324,15 @@ def bench_served(args, torch, T):
                        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"]},
     }
+    # Slices of 2^16 indices: requests above 2^16 span several steps (the
+    # continuous-batching path proper); saturation and latency beside the
+    # one-step configuration above.
+    lat, wall, st = run(np.zeros(R), slice_indices=1 << 16)
+    out["saturation_slice_2p16"] = {
+        "req_per_s": round(R / wall, 1), "gbs": round(total_bytes / wall / 1e9, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"],
+        "requests_spanning_steps": int((sizes > (1 << 16)).sum())}
     rng = np.random.default_rng(args.seed + 13)
     for frac in (0.25, 0.5, 0.8, 0.95):
         rate = frac * sat_rate
@@ -331,8 +358,117 @@ def bench_served(args, torch, T):
     return out
 
 
+def _counter_rows(directory: str):
+    """Rows of every rocprofv3 counter_collection.csv under directory."""
+    import csv
+    import glob
+
+    rows = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)):
+        with open(path, newline="") as f:
+            rows.extend(csv.DictReader(f))
+    if not rows:
+        raise SystemExit(f"no *counter_collection.csv under {directory}")
+    return rows
+
+
+def _per_leg(rows, counter: str, n_legs: int):
+    """Counter value summed over the library's dispatches of each leg: a
+    marker (k_memcpy16 of grid size 256) opens the next leg; dispatches
+    before the first marker (input set-up) and outside the library (torch's
+    own kernels) are not counted."""
+    disp = {}
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        d = int(r["Dispatch_Id"])
+        rec = disp.setdefault(d, [r["Kernel_Name"], int(r["Grid_Size"]), 0.0])
+        rec[2] += float(r["Counter_Value"])
+    out = [0.0] * n_legs
+    leg = -1
+    for d in sorted(disp):
+        name, grid, value = disp[d]
+        if "k_memcpy16" in name and grid == 256:
+            leg += 1
+            continue
+        if leg >= 0 and "trove_amd::" in name:
+            if leg >= n_legs:
+                raise SystemExit("more marked legs in the counter run than in the leg list")
+            out[leg] += value
+    if leg + 1 != n_legs:
+        raise SystemExit(f"{leg + 1} marked legs in the counter run, {n_legs} in the leg list")
+    return out
+
+
+def traffic_from(fetch_dir: str, write_dir: str, legs_json: str) -> int:
+    """Builds profiles/traffic.json (HBM bytes per launch of every bench leg)
+    from FETCH_SIZE and WRITE_SIZE counter runs of `bench.py --profile-legs`.
+    Calibration: the memcpy_ceiling leg copies a known number of bytes each
+    way; FETCH_SIZE and WRITE_SIZE are scaled by what they read for it."""
+    import subprocess
+
+    with open(legs_json) as f:
+        legs = json.load(f)["legs"]
+    fetch = _per_leg(_counter_rows(fetch_dir), "FETCH_SIZE", len(legs))
+    write = _per_leg(_counter_rows(write_dir), "WRITE_SIZE", len(legs))
+    per = {}
+    for (name, nbytes), fk, wk in zip(legs, fetch, write):
+        rec = per.setdefault(name, [0, 0.0, 0.0, nbytes])
+        rec[0] += 1
+        rec[1] += fk
+        rec[2] += wk
+    if "memcpy_ceiling" not in per:
+        raise SystemExit("the leg list has no memcpy_ceiling leg to calibrate on")
+    c = per["memcpy_ceiling"]
+    each_way = c[3] / 2
+    f_fac = each_way / (c[1] / c[0] * 1024)
+    w_fac = each_way / (c[2] / c[0] * 1024)
+    try:
+        commit = subprocess.run(["git", "-C", ROOT, "log", "-1", "--format=%h", "--", "trove_amd/csrc", "include"],
+                                capture_output=True, text=True, timeout=20).stdout.strip() or None
+    except (OSError, subprocess.SubprocessError):
+        commit = None
+    kernels, table = {}, []
+    for name in sorted(per):
+        n_, fk, wk, nbytes = per[name]
+        fk, wk = fk / n_, wk / n_
+        rd, wr = fk * 1024 * f_fac, wk * 1024 * w_fac
+        kernels[name] = int(round(rd + wr))
+        table.append(f"| `{name}` | {fk:.0f} | {wk:.0f} | {rd / 1e9:.3f} | {wr / 1e9:.3f} | {nbytes / 1e9:.3f} | "
+                     f"{(rd + wr) / nbytes:.2f} |")
+    out = {"unit": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated on the plain copy)",
+           "kernel_sources_sha256": kernel_sources_sha256(), "kernels_commit": commit,
+           "calibration": {"fetch_factor": round(f_fac, 4), "write_factor": round(w_fac, 4),
+                           "bytes_each_way": int(each_way)},
+           "kernels": kernels}
+    with open(TRAFFIC_PATH, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+        f.write("\n")
+    print(f"Kernel sources sha256 {out['kernel_sources_sha256'][:16]} (commit {commit}); calibration on the "
+          f"memcpy_ceiling leg ({int(each_way)} bytes each way): FETCH_SIZE x {f_fac:.3f}, WRITE_SIZE x {w_fac:.3f}.")
+    print()
+    print("| leg | FETCH KiB | WRITE KiB | read GB (cal.) | written GB (cal.) | algorithmic GB | HBM / algorithmic |")
+    print("|---|---|---|---|---|---|---|")
+    print("\n".join(table))
+    return 0
+
+
+def load_traffic():
+    """(kernels, stale): the committed per-leg HBM bytes, and whether they were
+    measured on other kernel sources than these (then kernels is None)."""
+    if not os.path.exists(TRAFFIC_PATH):
+        return None, True
+    with open(TRAFFIC_PATH) as f:
+        tr = json.load(f)
+    if tr.get("kernel_sources_sha256") != kernel_sources_sha256():
+        return None, True
+    return tr.get("kernels", {}), False
+
+
 def main():
     args = parse_args()
+    if args.traffic_from:
+        return traffic_from(*args.traffic_from)
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))

@@ -2,7 +2,8 @@
 //
 // Public interface of trove/memory.h:136-168: warp_load / warp_store move
 // element k of the array at ptr[offset + k * stride], and uncoalesced_store
-// writes the array contiguously from ptr with the given stride. When every
+// writes element k to ptr[k]: its stride argument is accepted and ignored,
+// as in the reference (memory.h:59-98 advances the offset by 1). When every
 // lane of a tile passes offset = its rank and stride = the tile width, each
 // element access is one coalesced wavefront instruction (global_load_dword,
 // _dwordx2 or _dwordx4 depending on the element width).

@@ -103,15 +103,20 @@ int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int
                   void* stream);
 
 /* dst[i] = counting_array(i * k) (k = struct_bytes / 4 words): the store
- * benchmark of tests/benchmark.cu:52-72 (SHFL: store_warp_contiguous;
- * DIRECT: plain stores). struct_bytes must be a multiple of 4 and at most
- * 256 (else TROVE_ERR_SIZE). */
-int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, void* stream);
+ * benchmark of tests/benchmark.cu:52-72 (SHFL: store_warp_contiguous<tile>;
+ * DIRECT: plain stores). tile is the reference program's TILE_SIZE
+ * (tests/Makefile:4, tests/benchmark.cu:47-50; its default there is 32):
+ * 0 = 64, else a power of two 2..64 (TROVE_ERR_VARIANT otherwise); DIRECT
+ * ignores it. struct_bytes must be a multiple of 4 and at most 256 (else
+ * TROVE_ERR_SIZE). */
+int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, int tile, void* stream);
 
 /* out[i] = sum of the 32-bit words of src[i] (wrapping): the load benchmark
- * of tests/benchmark.cu:74-88. struct_bytes must be a multiple of 4 and at
- * most 256 (else TROVE_ERR_SIZE). */
-int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, int variant, void* stream);
+ * of tests/benchmark.cu:74-88 (SHFL: load_warp_contiguous<tile>, tile as for
+ * trove_store_counting). struct_bytes must be a multiple of 4 and at most
+ * 256 (else TROVE_ERR_SIZE). */
+int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, int variant, int tile,
+                   void* stream);
 
 /* Block interface copy over n_elems 32-bit elements, s per lane
  * (s in 1..16): a grid-stride loop over n_elems / s arrays that calls
@@ -138,6 +143,18 @@ int trove_memcpy_ceiling(const void* src, void* dst, int64_t bytes, void* stream
  * counterpart (benchmark plumbing like trove_memcpy_ceiling). */
 int trove_random_lines_ceiling(const void* buf, int64_t n_lines, const uint32_t* line_idx, uint32_t* out,
                                int64_t n_reads, void* stream);
+
+/* Random row writes: for r < n_writes, row min(row_idx[r], n_rows - 1) of
+ * buf (rows of row_bytes = 32, 60, 64 or 128 bytes; TROVE_ERR_SIZE
+ * otherwise) receives the 32-bit words r * (row_bytes / 4) + j, j <
+ * row_bytes / 4, written as 16-byte lane words (4-byte ones for 60-byte
+ * rows) with many rows in flight per wavefront, as ordinary stores or, with
+ * nontemporal != 0, nontemporal ones: the rate at which the device absorbs
+ * random row writes, the ceiling of scatters of that row size. Rows should
+ * be distinct (else which write lands is unspecified). No reference
+ * counterpart (benchmark plumbing like trove_memcpy_ceiling). */
+int trove_random_rows_write_ceiling(void* buf, int64_t n_rows, int row_bytes, const int32_t* row_idx,
+                                    int64_t n_writes, int nontemporal, void* stream);
 
 /* ---- Batched jobs: many independent gathers / copies in one persistent launch ----
  * No reference counterpart (the reference launches one kernel per
@@ -191,13 +208,20 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
                        int32_t max_slots, int64_t slice_indices, int32_t variant, void* stream,
                        trove_serve** out);
 /* Queues a request. Host indices are range-checked (TROVE_ERR_RANGE) and
- * must stay valid until the request is done; device indices are trusted.
- * TROVE_ERR_BUDGET when n rows exceed the whole budget. */
+ * copied before the call returns, so the caller's array may change or be
+ * freed right after. Device indices are read in place and must stay valid
+ * until the request is done; the step kernels check each of them against
+ * the table as they gather (an index outside [0, n_rows) reads row 0 and is
+ * never dereferenced past the table), and such a request completes as
+ * rejected: trove_serve_poll, _output and _fetch then return
+ * TROVE_ERR_RANGE. TROVE_ERR_BUDGET when n rows exceed the whole budget. */
 int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on_device, uint64_t* request_id);
 /* Retires finished steps, admits queued requests and launches one step.
  * *in_flight = requests queued or running plus steps outstanding. */
 int trove_serve_step(trove_serve* s, int32_t* in_flight);
-/* *done = 1 once the request's rows are written; latency from submission. */
+/* *done = 1 once the request's rows are written; latency from submission.
+ * Returns TROVE_ERR_RANGE (with *done = 1) for a completed request that
+ * held a device index outside the table. */
 int trove_serve_poll(trove_serve* s, uint64_t request_id, int32_t* done, double* latency_s);
 /* Device address of a finished request's n output rows (until released). */
 int trove_serve_output(trove_serve* s, uint64_t request_id, void** dev_out);
@@ -212,7 +236,8 @@ int trove_serve_stats(const trove_serve* s, int64_t* completed, int64_t* indices
                       int32_t* queued, int32_t* active);
 /* Replays an arrival trace in real time: request i (idx[i], n[i] rows) is
  * submitted at arrival_s[i] seconds after the start (ascending); returns
- * when all are done, with latency_s[i] = completion - scheduled arrival.
+ * when all are done, with latency_s[i] = completion - scheduled arrival
+ * (-1 for a request rejected for a device index outside the table).
  * release_on_done frees each request's rows as soon as it completes. */
 int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
                        const int64_t* n, int idx_on_device, int release_on_done, double* latency_s);

@@ -48,6 +48,11 @@ def ref_lib():
     L.ref_rotate.restype = ctypes.c_int
     L.ref_counting_sum.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p]
     L.ref_counting_sum.restype = ctypes.c_int
+    for name in ("ref_mod_inverse_r_table", "ref_bits_table", "ref_inverse_table"):
+        getattr(L, name).argtypes = [_i32p]
+        getattr(L, name).restype = None
+    L.ref_warp_memory.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p]
+    L.ref_warp_memory.restype = ctypes.c_int
     return L
 
 
@@ -77,6 +82,17 @@ def lib():
         for name in ("oracle_gcd", "oracle_mod_inverse"):
             getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_int]
             getattr(L, name).restype = ctypes.c_int
+        L.oracle_mod_inverse_from.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_mod_inverse_from.restype = ctypes.c_int
+        L.oracle_pow2_odd_bits.argtypes = [ctypes.c_int]
+        L.oracle_pow2_odd_bits.restype = ctypes.c_int
+        L.oracle_inverse.argtypes = [_i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_inverse.restype = ctypes.c_int
+        L.oracle_warp_load.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u8p]
+        L.oracle_warp_store.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, _u8p, ctypes.c_int, ctypes.c_int]
+        L.oracle_uncoalesced_store.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, _u8p, ctypes.c_int]
+        for name in ("oracle_warp_load", "oracle_warp_store", "oracle_uncoalesced_store"):
+            getattr(L, name).restype = None
         L.oracle_floor_log2.argtypes = [ctypes.c_int]
         L.oracle_floor_log2.restype = ctypes.c_int
         L.oracle_schema.argtypes = [ctypes.c_int]
@@ -165,3 +181,30 @@ def c2r_tile(d: np.ndarray) -> np.ndarray:
     if lib().oracle_c2r_tile(_p(d, _i32p), n, m) != 0:
         raise ValueError("unsupported tile shape")
     return d
+
+
+def inverse(fn, x: int, p: int = 0) -> int:
+    """First p' >= p with fn[p'] == x (-1 if none), fn an int sequence."""
+    fn = np.ascontiguousarray(fn, dtype=np.int32)
+    return lib().oracle_inverse(_p(fn, _i32p), fn.size, x, p)
+
+
+def warp_memory(src: np.ndarray, eb: int, s: int, offset, stride, stored_init: np.ndarray):
+    """The reference's strided per-lane access for a tile of lanes: lane l's
+    array is warp_load(src, offset[l], stride[l]); returns (loaded, stored,
+    uncoalesced): loaded[l] = that array, stored = stored_init after every
+    lane's warp_store(array, stored, offset[l], stride[l]) in lane order, and
+    uncoalesced[l] = uncoalesced_store(array, row l) (uint8 arrays)."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    lanes = len(offset)
+    loaded = np.zeros((lanes, s * eb), dtype=np.uint8)
+    stored = np.array(stored_init, dtype=np.uint8, copy=True)
+    unc = np.zeros((lanes, s * eb), dtype=np.uint8)
+    L = lib()
+    for l in range(lanes):
+        row = loaded[l]
+        L.oracle_warp_load(_p(src, _u8p), eb, s, int(offset[l]), int(stride[l]), row.ctypes.data_as(_u8p))
+        L.oracle_warp_store(row.ctypes.data_as(_u8p), eb, s, _p(stored, _u8p), int(offset[l]), int(stride[l]))
+        urow = unc[l]
+        L.oracle_uncoalesced_store(row.ctypes.data_as(_u8p), eb, s, urow.ctypes.data_as(_u8p), 3)
+    return loaded, stored, unc

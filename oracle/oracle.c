@@ -11,7 +11,9 @@
  *   1. semantics of the access paths as scalar loops (copy, gather, scatter,
  *      the benchmark's counting store and summing load, the block interface);
  *   2. the reference's compile-time number theory (gcd, modular inverse,
- *      floor log2) and its runtime rotation;
+ *      floor log2, power-of-two / odd tests, inverse of a map), its strided
+ *      per-lane memory access (warp_load / warp_store / uncoalesced_store)
+ *      and its runtime rotation;
  *   3. the reference's R2C / C2R lane schedule (three schemas: odd,
  *      power-of-two, composite) simulated for a whole tile of n lanes;
  *   4. OpenMP versions of copy and gather (the CPU baseline).
@@ -114,6 +116,46 @@ int oracle_mod_inverse(int a, int m) {
     for (int r = 1; r < m; ++r)
         if ((r * a) % m == 1) return r;
     return -1;
+}
+
+/* static_mod_inverse<a, m, r> with an explicit start (static_mod_inverse.h:
+ * 32-52 recurses r, r + 1, ... and stops with -1 at r == m): the first
+ * r' = r, ..., m - 1 with (r' * a) % m == 1, else -1. */
+int oracle_mod_inverse_from(int a, int m, int r) {
+    for (; r < m; ++r)
+        if ((r * a) % m == 1) return r;
+    return -1;
+}
+
+/* is_power_of_two / is_odd (utility.h:96-104): (m & (m - 1)) == 0 and
+ * (m & 1) == 1, returned as bit 0 and bit 1 of one value. */
+int oracle_pow2_odd_bits(int m) { return ((m & (m - 1)) == 0) + 2 * ((m & 1) == 1); }
+
+/* inverse<T, Fn, x, p> (utility.h:121-126): the first p' = p, p + 1, ...
+ * with Fn(p') == x, for Fn given as a table fn[0..n_fn); -1 when none lies
+ * in the table. */
+int oracle_inverse(const int32_t* fn, int n_fn, int x, int p) {
+    for (; p < n_fn; ++p)
+        if (fn[p] == x) return p;
+    return -1;
+}
+
+/* warp_load / warp_store / uncoalesced_store (memory.h:101-168) for one
+ * lane's array of s elements of eb bytes: element k of the array is
+ * ptr[offset + k * stride] for warp_load / warp_store, and ptr[k] for
+ * uncoalesced_store (memory.h:59-98 advances its offset by 1 whatever the
+ * stride). */
+void oracle_warp_load(const uint8_t* ptr, int eb, int s, int offset, int stride, uint8_t* out) {
+    for (int k = 0; k < s; ++k) memcpy(out + (int64_t)k * eb, ptr + ((int64_t)offset + (int64_t)k * stride) * eb, (size_t)eb);
+}
+
+void oracle_warp_store(const uint8_t* in, int eb, int s, uint8_t* ptr, int offset, int stride) {
+    for (int k = 0; k < s; ++k) memcpy(ptr + ((int64_t)offset + (int64_t)k * stride) * eb, in + (int64_t)k * eb, (size_t)eb);
+}
+
+void oracle_uncoalesced_store(const uint8_t* in, int eb, int s, uint8_t* ptr, int stride) {
+    (void)stride;
+    for (int k = 0; k < s; ++k) memcpy(ptr + (int64_t)k * eb, in + (int64_t)k * eb, (size_t)eb);
 }
 
 /* static_log (utility.h:80-94): floor(log2 m) for m >= 1. */

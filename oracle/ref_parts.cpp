@@ -1,8 +1,9 @@
 // C ABI over the parts of the reference that compile unmodified on this
 // image: its compile-time number theory (trove/static_gcd.h,
-// trove/static_mod_inverse.h, trove/utility.h static_log), its register
-// array and runtime rotation (trove/array.h, trove/rotate.h) and its test
-// helpers (counting_array, sum; trove/utility.h:33-78). Built by
+// trove/static_mod_inverse.h, trove/utility.h static_log, is_power_of_two,
+// is_odd, inverse), its register array, strided per-lane memory access and
+// runtime rotation (trove/array.h, trove/memory.h, trove/rotate.h) and its
+// test helpers (counting_array, sum; trove/utility.h:33-78). Built by
 // oracle/Makefile (target `ref`) from the reference sources in place, host
 // code only, into oracle/_ref/libref_parts.so. TEST INFRASTRUCTURE ONLY:
 // tests/test_oracle.py compares the oracle against these tables.
@@ -10,6 +11,7 @@
 #include <trove/static_mod_inverse.h>
 #include <trove/utility.h>
 #include <trove/rotate.h>
+#include <trove/memory.h>
 
 #include <utility>
 
@@ -53,9 +55,74 @@ int counting_sum(int v, int step, int* words) {
     return trove::sum(x);
 }
 
+// static_mod_inverse<a, m, r> for r <= m (the reference's recursion only
+// stops at r == m); -2 marks r > m.
+template <int R, int A, int... M>
+void inv_r_row(int* out, std::integer_sequence<int, M...>) {
+    ((out[((R - 1) * kN + A) * 32 + M] = (R <= M + 1 ? trove::static_mod_inverse<A, M + 1, (R <= M + 1 ? R : M + 1)>::value : -2)), ...);
+}
+template <int R, int... A>
+void inv_r_block(int* out, std::integer_sequence<int, A...>) {
+    (inv_r_row<R, A>(out, std::make_integer_sequence<int, 32>{}), ...);
+}
+
+template <int... M>
+void bits_table(int* out, std::integer_sequence<int, M...>) {
+    ((out[M] = int(trove::is_power_of_two<M>::value) + 2 * int(trove::is_odd<M>::value)), ...);
+}
+
+template <int p>
+struct affine64 {
+    static const int value = (5 * p + 3) % 64;
+};
+template <int P0, int... X>
+void inverse_row(int* out, std::integer_sequence<int, X...>) {
+    ((out[X] = trove::inverse<int, affine64, X, P0>::value), ...);
+}
+
+template <int S>
+void memory_one(const int* src, int offset, int stride, int* loaded, int* stored, int* unc) {
+    typedef trove::array<int, S> A;
+    const A a = trove::warp_load<A>(src, offset, stride);
+    trove::make_carray<int, S>(a, loaded);
+    trove::warp_store(a, stored, offset, stride);
+    trove::uncoalesced_store(a, unc, 3);
+}
+
 }  // namespace
 
 extern "C" {
+
+// out[((r - 1) * 33 + a) * 32 + m - 1] = static_mod_inverse<a, m, r>::value
+// for 1 <= r <= 6, 0 <= a <= 32, 1 <= m <= 32 (-2 where r > m).
+void ref_mod_inverse_r_table(int* out) {
+    inv_r_block<1>(out, std::make_integer_sequence<int, kN>{});
+    inv_r_block<2>(out, std::make_integer_sequence<int, kN>{});
+    inv_r_block<3>(out, std::make_integer_sequence<int, kN>{});
+    inv_r_block<4>(out, std::make_integer_sequence<int, kN>{});
+    inv_r_block<5>(out, std::make_integer_sequence<int, kN>{});
+    inv_r_block<6>(out, std::make_integer_sequence<int, kN>{});
+}
+// out[m] = is_power_of_two<m> + 2 * is_odd<m>, 0 <= m <= 256.
+void ref_bits_table(int* out) { bits_table(out, std::make_integer_sequence<int, 257>{}); }
+// out[x] = inverse<int, (5p + 3) % 64, x>::value, out[64 + x] the same
+// search from p = 10, 0 <= x < 64.
+void ref_inverse_table(int* out) {
+    inverse_row<0>(out, std::make_integer_sequence<int, 64>{});
+    inverse_row<10>(out + 64, std::make_integer_sequence<int, 64>{});
+}
+// One lane of trove/memory.h on int arrays of s elements (s in 1, 2, 5, 16):
+// loaded = warp_load(src, offset, stride), warp_store of it into stored, and
+// uncoalesced_store of it to unc with stride 3; returns -1 for another s.
+int ref_warp_memory(int s, const int* src, int offset, int stride, int* loaded, int* stored, int* unc) {
+    switch (s) {
+        case 1: memory_one<1>(src, offset, stride, loaded, stored, unc); return 0;
+        case 2: memory_one<2>(src, offset, stride, loaded, stored, unc); return 0;
+        case 5: memory_one<5>(src, offset, stride, loaded, stored, unc); return 0;
+        case 16: memory_one<16>(src, offset, stride, loaded, stored, unc); return 0;
+        default: return -1;
+    }
+}
 
 // out[u * 33 + v] = static_gcd<u, v>::value, 1 <= u, v <= 32.
 void ref_gcd_table(int* out) { gcd_table(out, std::make_integer_sequence<int, kN - 1>{}); }

@@ -335,3 +335,153 @@ int devtest_mixed(int s, uint64_t mask, const void* base, const int* perm, const
 }
 
 }  // extern "C"
+
+// ---- trove/memory.h and the compile-time number theory ----
+#include <trove/memory.h>
+#include <trove/static_gcd.h>
+#include <trove/static_mod_inverse.h>
+#include <trove/utility.h>
+
+#include <utility>
+
+namespace {
+
+// Lane l: a = warp_load<array<T, S>>(src, offset[l], stride[l]) (element k
+// from src[offset[l] + k * stride[l]]), written out as loaded[l * S + k];
+// warp_store(a, stored, offset[l], stride[l]); uncoalesced_store(a,
+// unc + l * S, 3) (element k to unc[l * S + k]; the stride is ignored). VOL
+// takes the volatile-pointer overloads.
+template <typename T, int S, bool VOL>
+__global__ __launch_bounds__(64) void k_memory(const T* src, const int* offset, const int* stride, T* loaded,
+                                               T* stored, T* unc) {
+    typedef trove::array<T, S> A;
+    const int lane = threadIdx.x;
+    A a;
+    if constexpr (VOL) {
+        a = trove::warp_load<A>(static_cast<const volatile T*>(src), offset[lane], stride[lane]);
+    } else {
+        a = trove::warp_load<A>(src, offset[lane], stride[lane]);
+    }
+    T out[S];
+    trove::make_carray<T, S>(a, out);
+    for (int k = 0; k < S; ++k) loaded[lane * S + k] = out[k];
+    trove::warp_store(a, stored, offset[lane], stride[lane]);
+    if constexpr (VOL) {
+        trove::uncoalesced_store(a, static_cast<volatile T*>(unc + lane * S), 3);
+    } else {
+        trove::uncoalesced_store(a, unc + lane * S, 3);
+    }
+}
+
+// Tables of the public compile-time helpers, evaluated by the product
+// templates at compile time and copied out by a kernel.
+template <int N>
+struct itable {
+    int v[N];
+};
+
+constexpr int kNT = 65;  // arguments 0..64
+
+template <size_t... I>
+constexpr itable<sizeof...(I)> mod_inverse_tab(std::index_sequence<I...>) {
+    // entry a * 64 + (m - 1) = static_mod_inverse<a, m>, a in [0, 64], m in [1, 64]
+    return {{trove::static_mod_inverse<int(I / 64), int(I % 64) + 1>::value...}};
+}
+template <size_t... I>
+constexpr itable<sizeof...(I)> mod_inverse_r_tab(std::index_sequence<I...>) {
+    // entry (r - 1) * 33 * 32 + a * 32 + (m - 1) = static_mod_inverse<a, m, r>,
+    // r in [1, 6], a in [0, 32], m in [1, 32]
+    return {{trove::static_mod_inverse<int((I / 32) % 33), int(I % 32) + 1, int(I / (33 * 32)) + 1>::value...}};
+}
+template <size_t... I>
+constexpr itable<sizeof...(I)> gcd_tab(std::index_sequence<I...>) {
+    // entry u * 65 + v = static_gcd<u, v>, u, v in [0, 64]
+    return {{trove::static_gcd<int(I / kNT), int(I % kNT)>::value...}};
+}
+template <size_t... I>
+constexpr itable<sizeof...(I)> log_tab(std::index_sequence<I...>) {
+    // entry m - 1 = static_log<m>, m in [1, 1024]
+    return {{trove::static_log<int(I) + 1>::value...}};
+}
+template <size_t... I>
+constexpr itable<sizeof...(I)> bits_tab(std::index_sequence<I...>) {
+    // entry m = is_power_of_two<m> + 2 * is_odd<m>, m in [0, 1024]
+    return {{(int(trove::is_power_of_two<int(I)>::value) + 2 * int(trove::is_odd<int(I)>::value))...}};
+}
+// A compile-time map on [0, 64) for inverse<>: p -> (5 p + 3) mod 64 (a
+// bijection, so every x has a preimage in every run of 64 arguments).
+template <int p>
+struct affine64 {
+    static constexpr int value = (5 * p + 3) % 64;
+};
+template <size_t... I>
+constexpr itable<sizeof...(I)> inverse_tab(std::index_sequence<I...>) {
+    // entry x = inverse<int, affine64, x>, then entry 64 + x with the search
+    // starting at p = 10
+    return {{trove::inverse<int, affine64, int(I % 64), (I < 64 ? 0 : 10)>::value...}};
+}
+
+constexpr int kInvN = 65 * 64, kInvRN = 6 * 33 * 32, kGcdN = kNT * kNT, kLogN = 1024, kBitsN = 1025,
+              kInverseN = 128;
+__constant__ itable<kInvN> c_inv = mod_inverse_tab(std::make_index_sequence<kInvN>{});
+__constant__ itable<kInvRN> c_inv_r = mod_inverse_r_tab(std::make_index_sequence<kInvRN>{});
+__constant__ itable<kGcdN> c_gcd = gcd_tab(std::make_index_sequence<kGcdN>{});
+__constant__ itable<kLogN> c_log = log_tab(std::make_index_sequence<kLogN>{});
+__constant__ itable<kBitsN> c_bits = bits_tab(std::make_index_sequence<kBitsN>{});
+__constant__ itable<kInverseN> c_inverse = inverse_tab(std::make_index_sequence<kInverseN>{});
+
+// out: [inv | inv_r | gcd | log | bits | inverse], in that order.
+__global__ __launch_bounds__(256) void k_static_tables(int* out) {
+    for (int i = threadIdx.x; i < kInvN; i += 256) out[i] = c_inv.v[i];
+    out += kInvN;
+    for (int i = threadIdx.x; i < kInvRN; i += 256) out[i] = c_inv_r.v[i];
+    out += kInvRN;
+    for (int i = threadIdx.x; i < kGcdN; i += 256) out[i] = c_gcd.v[i];
+    out += kGcdN;
+    for (int i = threadIdx.x; i < kLogN; i += 256) out[i] = c_log.v[i];
+    out += kLogN;
+    for (int i = threadIdx.x; i < kBitsN; i += 256) out[i] = c_bits.v[i];
+    out += kBitsN;
+    for (int i = threadIdx.x; i < kInverseN; i += 256) out[i] = c_inverse.v[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+// Element bytes eb (4, 8 or 16) and array length s; src holds the words
+// warp_load reads, offset / stride one int per lane; loaded and unc are
+// 64 * s elements, stored must cover every offset + k * stride. vol != 0:
+// the volatile overloads (4- and 8-byte elements). Returns -1 for an
+// unsupported combination, -2 on a launch error.
+int devtest_memory(int eb, int s, int vol, const void* src, const int* offset, const int* stride, void* loaded,
+                   void* stored, void* unc) {
+#define MEM(E, T, S, V)                                                                                       \
+    if (eb == E && s == S && (vol != 0) == V) {                                                              \
+        hipLaunchKernelGGL((k_memory<T, S, V>), dim3(1), dim3(64), 0, 0, (const T*)src, offset, stride,      \
+                           (T*)loaded, (T*)stored, (T*)unc);                                                  \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                                                \
+    }
+    MEM(4, uint32_t, 1, false) MEM(4, uint32_t, 2, false) MEM(4, uint32_t, 5, false) MEM(4, uint32_t, 16, false)
+    MEM(8, uint64_t, 3, false) MEM(16, w16, 2, false) MEM(16, w16, 5, false)
+    MEM(4, uint32_t, 5, true) MEM(8, uint64_t, 3, true)
+#undef MEM
+    return -1;
+}
+
+// Sizes of the sections devtest_static_tables writes, in order.
+void devtest_static_table_sizes(int* sizes) {
+    sizes[0] = kInvN;
+    sizes[1] = kInvRN;
+    sizes[2] = kGcdN;
+    sizes[3] = kLogN;
+    sizes[4] = kBitsN;
+    sizes[5] = kInverseN;
+}
+
+int devtest_static_tables(int* out) {
+    hipLaunchKernelGGL(k_static_tables, dim3(1), dim3(256), 0, 0, out);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -2;
+}
+
+}  // extern "C"

@@ -14,8 +14,13 @@
 // reference's lines.
 // Everything goes through the C ABI (include/trove_amd.h).
 //
-// Usage: trove_benchmark [--n N] [--lower L] [--upper U] [--iterations I] [--seed S]
-// (defaults: the reference's N = 80*8*100*256 structs, k = 1..16, 10 launches).
+// Usage: trove_benchmark [--n N] [--lower L] [--upper U] [--iterations I] [--seed S] [--tile T]
+//                        [--contiguous-only 1]
+// (defaults: the reference's N = 80*8*100*256 structs, k = 1..16, 10 launches;
+// tile 64, a whole wavefront). --tile is the reference's TILE_SIZE
+// (tests/Makefile:4, tests/benchmark.cu:47-50, 32 there by default): the tile
+// width of store_warp_contiguous / load_warp_contiguous in the contiguous
+// store and load benchmarks.
 // Exit status 0 when every result passed, 1 otherwise, 2 on an error.
 #include <hip/hip_runtime.h>
 #include <trove_amd.h>
@@ -85,7 +90,8 @@ float time_ms(F launch, int iterations) {
 
 struct options {
     int64_t n = int64_t(80) * 8 * 100 * 256;
-    int lower = 1, upper = 16, iterations = 10;
+    int lower = 1, upper = 16, iterations = 10, tile = 64;
+    bool contiguous_only = false;  // --contiguous-only 1: the store and load benchmarks alone
     uint64_t seed = 1;
 };
 
@@ -102,12 +108,12 @@ void report(const char* name, int k, double bytes_per_launch, int iterations, fl
 void contiguous_store(const options& o, const char* name, int k, int variant) {
     const int s = 4 * k;
     device_buffer r(size_t(o.n) * s);
-    const float ms = time_ms([&] { trove_ok(trove_store_counting(r.p, o.n, s, variant, nullptr), name); },
+    const float ms = time_ms([&] { trove_ok(trove_store_counting(r.p, o.n, s, variant, o.tile, nullptr), name); },
                              o.iterations);
     bool correct = true;
     if (variant != TROVE_VARIANT_DIRECT) {
         device_buffer g(r.bytes);
-        trove_ok(trove_store_counting(g.p, o.n, s, TROVE_VARIANT_DIRECT, nullptr), "gold store");
+        trove_ok(trove_store_counting(g.p, o.n, s, TROVE_VARIANT_DIRECT, 0, nullptr), "gold store");
         correct = equal(r, g);
     }
     report(name, k, double(s) * o.n, o.iterations, ms, correct);
@@ -118,15 +124,15 @@ void contiguous_store(const options& o, const char* name, int k, int variant) {
 void contiguous_load(const options& o, const char* name, int k, int variant) {
     const int s = 4 * k;
     device_buffer src(size_t(o.n) * s);
-    trove_ok(trove_store_counting(src.p, o.n, s, TROVE_VARIANT_DIRECT, nullptr), "fill");
+    trove_ok(trove_store_counting(src.p, o.n, s, TROVE_VARIANT_DIRECT, 0, nullptr), "fill");
     device_buffer r(size_t(o.n) * 4);
     const float ms = time_ms(
-        [&] { trove_ok(trove_load_sum(src.p, static_cast<uint32_t*>(r.p), o.n, s, variant, nullptr), name); },
+        [&] { trove_ok(trove_load_sum(src.p, static_cast<uint32_t*>(r.p), o.n, s, variant, o.tile, nullptr), name); },
         o.iterations);
     bool correct = true;
     if (variant != TROVE_VARIANT_DIRECT) {
         device_buffer g(r.bytes);
-        trove_ok(trove_load_sum(src.p, static_cast<uint32_t*>(g.p), o.n, s, TROVE_VARIANT_DIRECT, nullptr),
+        trove_ok(trove_load_sum(src.p, static_cast<uint32_t*>(g.p), o.n, s, TROVE_VARIANT_DIRECT, 0, nullptr),
                  "gold load");
         correct = equal(r, g);
     }
@@ -138,7 +144,7 @@ void random_access(const options& o, const device_buffer& perm, const char* name
                    int variant) {
     const int s = 4 * k;
     device_buffer src(size_t(o.n) * s);
-    trove_ok(trove_store_counting(src.p, o.n, s, TROVE_VARIANT_DIRECT, nullptr), "fill");
+    trove_ok(trove_store_counting(src.p, o.n, s, TROVE_VARIANT_DIRECT, 0, nullptr), "fill");
     device_buffer r(src.bytes);
     const int32_t* idx = static_cast<const int32_t*>(perm.p);
     auto run = [&](void* out, int v) {
@@ -169,13 +175,18 @@ options parse(int argc, char** argv) {
         else if (arg == "--upper") o.upper = std::atoi(v);
         else if (arg == "--iterations") o.iterations = std::atoi(v);
         else if (arg == "--seed") o.seed = std::strtoull(v, nullptr, 10);
+        else if (arg == "--tile") o.tile = std::atoi(v);
+        else if (arg == "--contiguous-only") o.contiguous_only = std::atoi(v) != 0;
         else {
             std::fprintf(stderr, "unknown option %s\n", arg.c_str());
             std::exit(2);
         }
     }
-    if (o.n <= 0 || o.n > INT32_MAX || o.lower < 1 || o.upper > 16 || o.lower > o.upper || o.iterations < 1) {
-        std::fprintf(stderr, "need 0 < n <= 2^31-1, 1 <= lower <= upper <= 16, iterations >= 1\n");
+    const bool tile_ok = o.tile >= 2 && o.tile <= 64 && (o.tile & (o.tile - 1)) == 0;
+    if (o.n <= 0 || o.n > INT32_MAX || o.lower < 1 || o.upper > 16 || o.lower > o.upper || o.iterations < 1 ||
+        !tile_ok) {
+        std::fprintf(stderr,
+                     "need 0 < n <= 2^31-1, 1 <= lower <= upper <= 16, iterations >= 1, tile a power of two 2..64\n");
         std::exit(2);
     }
     return o;
@@ -189,6 +200,7 @@ int main(int argc, char** argv) {
     for (int k = o.lower; k <= o.upper; ++k) contiguous_store(o, "Contiguous Direct Store", k, TROVE_VARIANT_DIRECT);
     for (int k = o.lower; k <= o.upper; ++k) contiguous_load(o, "Contiguous SHFL Load", k, TROVE_VARIANT_SHFL);
     for (int k = o.lower; k <= o.upper; ++k) contiguous_load(o, "Contiguous Direct Load", k, TROVE_VARIANT_DIRECT);
+    if (o.contiguous_only) return all_passed ? 0 : 1;
 
     // A uniform random permutation of [0, n) (the reference sorts rand() keys;
     // this is Fisher-Yates from a seeded generator).

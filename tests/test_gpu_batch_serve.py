@@ -218,3 +218,151 @@ def test_serve_admission_limits(trove, cuda):
     e0 = srv.submit(np.zeros(0, dtype=np.int32), 0, on_device=False)
     assert srv.poll(e0)[0]
     srv.close()
+
+
+def test_serve_rejects_device_index_outside_table(trove, cuda):
+    # A device-resident request holding indices >= n_rows (and < 0) completes
+    # as rejected (ERR_RANGE from poll / output / fetch) while its
+    # neighbours in the same steps return the oracle's rows; the bad request
+    # spans several steps (slices of 128).
+    n_rows, rb = 1 << 14, 32
+    host_table, table = _table(n_rows, rb, 21)
+    rng = np.random.default_rng(21)
+    good = [rng.integers(0, n_rows, size=int(n), dtype=np.int32) for n in (300, 1000, 64, 777)]
+    bad = rng.integers(0, n_rows, size=900, dtype=np.int32)
+    bad[[5, 640]] = [n_rows, -3]
+    d_good = [torch.from_numpy(g).cuda() for g in good]
+    d_bad = torch.from_numpy(bad).cuda()
+    srv = T.Server(table, n_rows, rb, budget_bytes=1 << 20, max_slots=8, slice_indices=128)
+    ids = [srv.submit(d, d.numel(), on_device=True) for d in d_good[:2]]
+    bad_id = srv.submit(d_bad, d_bad.numel(), on_device=True)
+    ids += [srv.submit(d, d.numel(), on_device=True) for d in d_good[2:]]
+    srv.drain()
+    for call in (lambda: srv.poll(bad_id), lambda: srv.fetch(bad_id, bad.size), lambda: srv.output(bad_id)):
+        with pytest.raises(T.TroveError) as e:
+            call()
+        assert e.value.code == T.ERR_RANGE
+    for i, g in zip(ids, good):
+        assert srv.poll(i)[0]
+        assert np.array_equal(srv.fetch(i, g.size), oracle.gather(host_table, g, rb))
+    srv.release(bad_id)  # its arena rows are returned like any other request's
+    # The server keeps serving after a rejection.
+    rid = srv.submit(d_good[1], d_good[1].numel(), on_device=True)
+    srv.drain()
+    assert np.array_equal(srv.fetch(rid, good[1].size), oracle.gather(host_table, good[1], rb))
+    srv.close()
+
+
+def test_serve_copies_host_indices_at_submit(trove, cuda):
+    # The caller may overwrite (or drop) its host index array right after
+    # submit: the request gathers what was submitted.
+    n_rows, rb = 1 << 12, 16
+    host_table, table = _table(n_rows, rb, 22)
+    srv = T.Server(table, n_rows, rb, budget_bytes=1 << 20, max_slots=4, slice_indices=128)
+    idx = np.random.default_rng(22).integers(0, n_rows, size=5000, dtype=np.int32)
+    want = oracle.gather(host_table, idx, rb)
+    rid = srv.submit(idx, idx.size, on_device=False)
+    idx[:] = 0
+    del idx
+    srv.drain()
+    assert np.array_equal(srv.fetch(rid, 5000), want)
+    srv.close()
+
+
+def test_serve_submit_validates_arrays(trove, cuda):
+    n_rows, rb = 1 << 10, 16
+    _, table = _table(n_rows, rb, 23)
+    srv = T.Server(table, n_rows, rb, budget_bytes=1 << 16, max_slots=4, slice_indices=128)
+    with pytest.raises(ValueError):
+        srv.submit(np.arange(10, dtype=np.int64), 10, on_device=False)  # int64 would be read as int32 halves
+    with pytest.raises(ValueError):
+        srv.submit(np.arange(20, dtype=np.int32)[::2], 10, on_device=False)  # not contiguous
+    with pytest.raises(ValueError):
+        srv.submit(np.arange(5, dtype=np.int32), 10, on_device=False)  # fewer than n
+    with pytest.raises(ValueError):
+        srv.submit(torch.arange(10, dtype=torch.int64, device=cuda), 10, on_device=True)
+    with pytest.raises(ValueError):
+        srv.submit(torch.arange(10, dtype=torch.int32), 10, on_device=True)  # host tensor
+    with pytest.raises(ValueError):
+        srv.submit(torch.arange(5, dtype=torch.int32, device=cuda), 10, on_device=True)
+    srv.close()
+
+
+# ---- the batched and served workloads at their full shapes ----
+
+def test_batched_config_full_shape(trove, cuda):
+    # 1,024 jobs, sizes 4..64 B, lengths log-uniform 2^10..2^22, three index
+    # patterns, variants and item sizes chosen by the autotuner: every job of
+    # the persistent launch equals torch indexing on the device and the
+    # looped launches; 8 jobs equal the oracle; graph replays repeat it.
+    from trove_amd import tune
+    from trove_amd.workloads import BatchedWorkload
+
+    W = BatchedWorkload(1024, 20260101, torch)
+    table = tune.autotune(reps=1, seed=5)
+    chosen = set()
+
+    def choose(s, p, L):
+        v, c = tune.choose(table, s, p, L)
+        chosen.add(v)
+        return v, c
+
+    ba = T.Batch(W.jobs(choose, W.dst_a))
+    bb = T.Batch(W.jobs(lambda s, p, L: tune.choose(table, s, p, L), W.dst_b))
+    ba.run("persistent")
+    bb.run("looped")
+    torch.cuda.synchronize()
+    assert len(chosen) >= 2 and W.lengths.min() >= 1 << 10 and W.lengths.max() <= 1 << 22
+    idx_long = W.idx.long()
+    for j in range(W.n_jobs):
+        w0, nw = W.job_words(j)
+        k, L = int(W.sizes[j]) // 4, int(W.lengths[j])
+        i0 = W.idx_off[j]
+        want = W.src[w0:w0 + nw].view(L, k)[idx_long[i0:i0 + L]]
+        assert torch.equal(W.dst_a[w0:w0 + nw].view(L, k), want), ("persistent", j)
+        assert torch.equal(W.dst_b[w0:w0 + nw], W.dst_a[w0:w0 + nw]), ("looped", j)
+    for j in list(range(0, W.n_jobs, W.n_jobs // 8))[:8]:
+        w0, nw = W.job_words(j)
+        want = oracle.gather(W.src[w0:w0 + nw].cpu().numpy().view(np.uint8), W.job_indices(j), int(W.sizes[j]))
+        assert np.array_equal(W.dst_a[w0:w0 + nw].cpu().numpy().view(np.uint8), want), ("oracle", j)
+    ref = W.dst_a.clone()
+    for replay in range(2):
+        W.dst_a.zero_()
+        torch.cuda.synchronize()
+        ba.run("graph")
+        torch.cuda.synchronize()
+        assert all(torch.equal(ref[i:i + (1 << 26)], W.dst_a[i:i + (1 << 26)])
+                   for i in range(0, ref.numel(), 1 << 26)), ("graph", replay)
+    ba.close()
+    bb.close()
+    del W, ref, idx_long
+    torch.cuda.empty_cache()
+
+
+def test_served_config_full_shape(trove, cuda):
+    # The 2^24 x 32-B table, a 4 GiB output budget, 64 requests of
+    # log-uniform 2^8..2^18 Zipf(1.1) indices on the device, slices of 2^16
+    # (so the larger requests span several steps): every request equals the
+    # oracle in the mix and alone.
+    from trove_amd.workloads import ServedWorkload
+
+    W = ServedWorkload(64, 20260101, torch)
+    assert (W.sizes > (1 << 16)).sum() >= 3, "some requests must span steps"
+    host_table = W.table.cpu().numpy().view(np.uint8)
+    want = [oracle.gather(host_table, W.request(i), W.row_bytes) for i in range(W.n_requests)]
+    srv = T.Server(W.table, W.n_rows, W.row_bytes, budget_bytes=4 << 30, max_slots=1024, slice_indices=1 << 16)
+    ids = [srv.submit(W.device_request(i), int(W.sizes[i]), on_device=True) for i in range(W.n_requests)]
+    srv.drain()
+    st = srv.stats()
+    assert st["completed"] == W.n_requests and st["steps"] >= 4
+    for i, rid in enumerate(ids):
+        assert np.array_equal(srv.fetch(rid, int(W.sizes[i])), want[i]), ("mix", i)
+        srv.release(rid)
+    for i in range(W.n_requests):
+        rid = srv.submit(W.device_request(i), int(W.sizes[i]), on_device=True)
+        srv.drain()
+        assert np.array_equal(srv.fetch(rid, int(W.sizes[i])), want[i]), ("alone", i)
+        srv.release(rid)
+    srv.close()
+    del W, host_table
+    torch.cuda.empty_cache()

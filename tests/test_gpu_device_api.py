@@ -11,7 +11,14 @@
 * trove::load / trove::store under arbitrary EXEC masks move exactly the
   active lanes' structs (the tile cascade, aos.h:258-290);
 * load_warp_contiguous_lds / store_warp_contiguous_lds (trove/lds.h, no
-  reference counterpart) put struct l in lane l and write it back.
+  reference counterpart) put struct l in lane l and write it back;
+* warp_load / warp_store / uncoalesced_store (memory.h:136-168) move the
+  words the oracle's restatement moves, for the coalesced pattern and for
+  arbitrary per-lane offsets and strides, plain and volatile overloads;
+* static_mod_inverse (with and without a start value), static_gcd,
+  static_log, is_power_of_two, is_odd and inverse (static_mod_inverse.h:
+  32-52, static_gcd.h, utility.h:80-126), evaluated by the product headers,
+  equal the oracle's tables.
 """
 import ctypes
 import os
@@ -45,6 +52,12 @@ def dt(cuda):
     L.devtest_lds.restype = i
     L.devtest_user_struct.argtypes = [vp, vp, vp, vp, vp]
     L.devtest_user_struct.restype = i
+    L.devtest_memory.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp]
+    L.devtest_memory.restype = i
+    L.devtest_static_table_sizes.argtypes = [vp]
+    L.devtest_static_table_sizes.restype = None
+    L.devtest_static_tables.argtypes = [vp]
+    L.devtest_static_tables.restype = i
     for f in (L.devtest_transpose, L.devtest_rotate, L.devtest_converged, L.devtest_shfl, L.devtest_cascade):
         f.restype = i
     return L
@@ -266,3 +279,70 @@ def test_dynamic_path_pointers_a_word_apart(dt, s):
                     off = int(shift[lane]) + int(perm[lane]) * s
                     want[lane] = buf[off:off + s].view(np.uint32)
             assert np.array_equal(got, want), (s, hex(mask), pattern)
+
+
+def _memory_case(rng, s, lanes, coalesced):
+    # Distinct positions across lanes: each lane owns a disjoint segment and
+    # walks it with its own stride >= 1 (or offset = lane, stride = 64).
+    if coalesced:
+        return np.arange(lanes, dtype=np.int32), np.full(lanes, lanes, dtype=np.int32), lanes * s
+    seg = 4 * s + 3
+    stride = rng.integers(1, 5, size=lanes)
+    lead = np.array([rng.integers(0, seg - (s - 1) * int(st)) for st in stride])
+    return ((rng.permutation(lanes) * seg + lead).astype(np.int32), stride.astype(np.int32), lanes * seg)
+
+
+@pytest.mark.parametrize("eb,s,vol", [(4, 1, 0), (4, 2, 0), (4, 5, 0), (4, 16, 0), (8, 3, 0), (16, 2, 0),
+                                      (16, 5, 0), (4, 5, 1), (8, 3, 1)])
+def test_warp_memory_api(dt, eb, s, vol):
+    import oracle
+
+    rng = np.random.default_rng(eb * 1000 + s * 10 + vol)
+    for coalesced in (True, False):
+        offset, stride, span = _memory_case(rng, s, 64, coalesced)
+        src = rng.integers(0, 256, size=span * eb, dtype=np.uint8)
+        init = rng.integers(0, 256, size=span * eb, dtype=np.uint8)
+        d_src, d_stored = torch.from_numpy(src).cuda(), torch.from_numpy(init.copy()).cuda()
+        d_loaded = torch.zeros(64 * s * eb, dtype=torch.uint8, device="cuda")
+        d_unc = torch.zeros(64 * s * eb, dtype=torch.uint8, device="cuda")
+        assert dt.devtest_memory(eb, s, vol, d_src.data_ptr(), torch.from_numpy(offset).cuda().data_ptr(),
+                                 torch.from_numpy(stride).cuda().data_ptr(), d_loaded.data_ptr(),
+                                 d_stored.data_ptr(), d_unc.data_ptr()) == 0
+        loaded, stored, unc = oracle.warp_memory(src, eb, s, offset, stride, init)
+        assert np.array_equal(d_loaded.cpu().numpy().reshape(64, -1), loaded), (eb, s, vol, coalesced, "load")
+        assert np.array_equal(d_stored.cpu().numpy(), stored), (eb, s, vol, coalesced, "warp_store")
+        assert np.array_equal(d_unc.cpu().numpy().reshape(64, -1), unc), (eb, s, vol, coalesced, "uncoalesced")
+
+
+def test_static_tables(dt):
+    import oracle
+
+    L = oracle.lib()
+    sizes = np.zeros(6, dtype=np.int32)
+    dt.devtest_static_table_sizes(sizes.ctypes.data)
+    out = torch.zeros(int(sizes.sum()), dtype=torch.int32, device="cuda")
+    assert dt.devtest_static_tables(out.data_ptr()) == 0
+    parts = np.split(out.cpu().numpy(), np.cumsum(sizes)[:-1])
+    inv, inv_r, gcd, log, bits, inverse = parts
+    inv = inv.reshape(65, 64)
+    for a in range(65):
+        for m in range(1, 65):
+            assert inv[a, m - 1] == L.oracle_mod_inverse(a, m), ("static_mod_inverse", a, m)
+    inv_r = inv_r.reshape(6, 33, 32)
+    for r in range(1, 7):
+        for a in range(33):
+            for m in range(1, 33):
+                assert inv_r[r - 1, a, m - 1] == L.oracle_mod_inverse_from(a, m, r), ("start", a, m, r)
+    gcd = gcd.reshape(65, 65)
+    for u in range(1, 65):
+        for v in range(1, 65):
+            assert gcd[u, v] == L.oracle_gcd(u, v), ("static_gcd", u, v)
+    # gcd(0, v) = v and gcd(u, 0) = u (the reference's zero cases are bool-typed).
+    assert np.array_equal(gcd[0, 1:], np.arange(1, 65)) and np.array_equal(gcd[1:, 0], np.arange(1, 65))
+    for m in range(1, 1025):
+        assert log[m - 1] == L.oracle_floor_log2(m), ("static_log", m)
+    for m in range(0, 1025):
+        assert bits[m] == L.oracle_pow2_odd_bits(m), ("is_power_of_two / is_odd", m)
+    fn = [(5 * p + 3) % 64 for p in range(200)]
+    for x in range(64):
+        assert inverse[x] == oracle.inverse(fn, x, 0) and inverse[64 + x] == oracle.inverse(fn, x, 10), x

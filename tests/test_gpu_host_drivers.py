@@ -22,12 +22,21 @@ def _run(args, timeout=240):
     return subprocess.run([exe, *args[1:]], capture_output=True, text=True, timeout=timeout)
 
 
-def test_benchmark_driver_results_pass(cuda):
-    # 2^20 + 37 structs: the last wavefront of every kernel is partial.
-    p = _run(["trove_benchmark", "--n", str((1 << 20) + 37), "--iterations", "2"])
+@pytest.mark.parametrize("tile", [None, "32", "16", "2"])
+def test_benchmark_driver_results_pass(cuda, tile):
+    # 2^20 + 37 structs: the last wavefront of every kernel is partial. tile:
+    # the reference program's TILE_SIZE (32 by default there, tests/Makefile:4)
+    # for the contiguous store and load benchmarks; None = 64.
+    args = ["trove_benchmark", "--n", str((1 << 20) + 37), "--iterations", "2"]
+    names = NAMES
+    if tile is not None:
+        # The tile width changes only the contiguous benchmarks: run those.
+        args += ["--tile", tile, "--contiguous-only", "1"]
+        names = NAMES[:4]
+    p = _run(args)
     assert p.returncode == 0, p.stdout[-3000:] + p.stderr[-3000:]
     lines = [ln.split(", ") for ln in p.stdout.strip().splitlines()]
-    assert [(ln[0], int(ln[1])) for ln in lines] == [(n, k) for n in NAMES for k in range(1, 17)]
+    assert [(ln[0], int(ln[1])) for ln in lines] == [(n, k) for n in names for k in range(1, 17)]
     assert all(ln[3] == "Results passed" and float(ln[2]) > 0 for ln in lines), p.stdout
 
 

@@ -264,6 +264,23 @@ def test_counting_store_and_load_sum(trove, cuda):
                 assert np.array_equal(host(out).view(np.uint32), oracle.load_sum(src, n, k)), (k, n, variant)
 
 
+@pytest.mark.parametrize("tile", [32, 16, 8, 4, 2])
+def test_counting_store_and_load_sum_tiles(trove, cuda, tile):
+    # The reference program's TILE_SIZE (tests/Makefile:4, 32 by default):
+    # store_warp_contiguous / load_warp_contiguous on sub-wavefront tiles,
+    # including structs of more words than the tile (the direct path there).
+    n = 64 * 37 + 29
+    for k in (1, 2, 3, 4, 5, 8, 15, 16, 24, 64):
+        s = 4 * k
+        d = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+        T.store_counting(d, n, s, variant="shfl", tile=tile)
+        assert np.array_equal(host(d).view(np.uint32), oracle.counting_store(n, k)), (k, tile)
+        src = np.random.default_rng(k * 7 + tile).integers(0, 2**32, size=n * k, dtype=np.uint64).astype(np.uint32)
+        out = torch.zeros(n, dtype=torch.int32, device=cuda)
+        T.load_sum(dev(src.view(np.uint8)), out, n, s, variant="shfl", tile=tile)
+        assert np.array_equal(host(out).view(np.uint32), oracle.load_sum(src, n, k)), (k, tile)
+
+
 @pytest.mark.parametrize("k", [1, 3, 4, 5, 8, 10, 13, 16, 24, 40, 64])
 def test_counting_store_and_load_sum_many_groups(trove, cuda, k):
     # Enough 64-struct groups that the launch spreads work-groups over all
@@ -340,6 +357,24 @@ def test_random_lines_ceiling_values(trove, cuda, n_reads):
     assert np.array_equal(host(out).view(np.uint32), want)
 
 
+@pytest.mark.parametrize("row_bytes", [32, 60, 64, 128])
+def test_random_rows_write_ceiling_values(trove, cuda, row_bytes):
+    # Write r lands at row idx[r] with the words r * k + j; untouched rows
+    # keep their contents (both store policies; partial last rounds).
+    k = row_bytes // 4
+    n_rows = 64 * 97 + 13
+    rng = np.random.default_rng(row_bytes)
+    for nt in (False, True):
+        for n_writes in (1, 100, n_rows):
+            idx = rng.permutation(n_rows)[:n_writes].astype(np.int32)
+            init = rng.integers(0, 2**32, size=n_rows * k, dtype=np.uint64).astype(np.uint32)
+            d_buf = dev(init.view(np.int32))
+            T.random_rows_write_ceiling(d_buf, n_rows, row_bytes, dev(idx), n_writes, nontemporal=nt)
+            want = init.reshape(n_rows, k).copy()
+            want[idx] = (np.arange(n_writes, dtype=np.uint64)[:, None] * k + np.arange(k)).astype(np.uint32)
+            assert np.array_equal(host(d_buf).view(np.uint32).reshape(n_rows, k), want), (nt, n_writes)
+
+
 def test_small_contiguous_config_bit_exact(trove, cuda):
     # Contiguous AoS copy, 20-byte structs, N = 2^20, against the scalar loop.
     n, s = 1 << 20, 20
@@ -360,40 +395,52 @@ def _equal_in_chunks(a: torch.Tensor, b: torch.Tensor, chunk: int = 1 << 26) -> 
     return all(torch.equal(a[i:i + chunk], b[i:i + chunk]) for i in range(0, a.numel(), chunk))
 
 
-@pytest.mark.parametrize("k", [1, 5, 8, 16])
-def test_full_size_sweep_properties(trove, cuda, k):
-    # Struct-size sweep at N = 2^26: copy is the identity; gathers through a
-    # seeded permutation agree with torch indexing everywhere (chunked) and
-    # with the oracle on 4096 sampled rows.
-    n, s = 1 << 26, 4 * k
-    d_src = torch.empty(n * k, dtype=torch.int32, device=cuda)
-    T.fill_words_device(d_src, n * k, 77 + k)
+@pytest.fixture(scope="module")
+def sweep_inputs(cuda):
+    """The struct-size sweep's inputs at N = 2^26: one 4 GiB source (the
+    first 2^26 * k words serve size k), a seeded permutation, both also on
+    the host for the oracle."""
+    n, kmax = 1 << 26, 16
+    d_src = torch.empty(n * kmax, dtype=torch.int32, device=cuda)
+    T.fill_words_device(d_src, d_src.numel(), 77)
+    perm = T.gen_permutation(n, 4242)
+    d_idx = dev(perm)
     d_dst = torch.empty_like(d_src)
+    src_host = host(d_src).view(np.uint8)
+    yield {"n": n, "d_src": d_src, "d_dst": d_dst, "d_idx": d_idx, "idx_long": d_idx.long(), "perm": perm,
+           "src_host": src_host}
+    del d_src, d_dst, d_idx
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("k", range(1, 17))
+def test_full_size_sweep_properties(trove, cuda, sweep_inputs, k):
+    # Struct-size sweep at N = 2^26, every k = 1..16: copies are the identity;
+    # gathers through the seeded permutation (tile path, coalesced_ptr, the
+    # LDS-staged path, and the naive kernel for a few sizes) agree with torch
+    # indexing on every row and with the oracle on 4096 sampled rows.
+    S = sweep_inputs
+    n, s = S["n"], 4 * k
+    d_src, d_dst = S["d_src"][:n * k], S["d_dst"][:n * k]
     for variant in ("shfl", "lds"):
         d_dst.zero_()
         T.copy(d_src, d_dst, n, s, variant=variant)
         torch.cuda.synchronize()
         assert _equal_in_chunks(d_dst, d_src), variant
-    perm = T.gen_permutation(n, 4242)
-    d_idx = dev(perm)
     rows = np.random.default_rng(k).integers(0, n, size=4096)
-    src_host = host(d_src).view(np.uint8)
-    want_rows = oracle.gather(src_host, perm[rows], s).view(np.int32).reshape(-1, k)
-    del src_host
+    want_rows = oracle.gather(S["src_host"][:n * s], S["perm"][rows], s).view(np.int32).reshape(-1, k)
     src2d = d_src.view(n, k)
     step = (1 << 26) // k
-    for variant in ("tile", "shfl", "direct", "lds"):
+    variants = ("tile", "shfl", "lds") + (("direct",) if k in (1, 5, 8, 16) else ())
+    for variant in variants:
         d_dst.zero_()
-        T.gather(d_src, d_idx, d_dst, n, s, variant=variant)
+        T.gather(d_src, S["d_idx"], d_dst, n, s, variant=variant)
         torch.cuda.synchronize()
         dst2d = d_dst.view(n, k)
         for r0 in range(0, n, step):
-            want = src2d[d_idx[r0:r0 + step].long()]
-            assert torch.equal(dst2d[r0:r0 + step], want), (variant, r0)
+            assert torch.equal(dst2d[r0:r0 + step], src2d[S["idx_long"][r0:r0 + step]]), (variant, r0)
         got_rows = dst2d[torch.from_numpy(rows).long().to(cuda)].cpu().numpy()
         assert np.array_equal(got_rows, want_rows), variant
-    del d_src, d_dst, d_idx
-    torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("s", [96, 256, 1024])
@@ -464,7 +511,7 @@ def test_gather_scatter_rows_near_the_index_limit(trove, cuda, s):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("s", [1, 2, 5, 15])
+@pytest.mark.parametrize("s", [1, 2, 3, 5, 7, 9, 11, 13, 15])
 def test_full_size_block_interface(trove, cuda, s):
     # The block-interface workload at its full shape: 2^28 32-bit elements,
     # s per lane (2^28 % s trailing elements), converged and with one group in

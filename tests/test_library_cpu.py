@@ -59,8 +59,8 @@ def test_bench_kernels_limited_to_256_bytes():
     L = T.lib()
     fake = 0x10000  # never dereferenced: the size check returns before any launch
     for b in (384, 512, 768, 1024):
-        assert L.trove_store_counting(fake, 100, b, 1, None) == T.ERR_SIZE
-        assert L.trove_load_sum(fake, fake, 100, b, 1, None) == T.ERR_SIZE
+        assert L.trove_store_counting(fake, 100, b, 1, 0, None) == T.ERR_SIZE
+        assert L.trove_load_sum(fake, fake, 100, b, 1, 0, None) == T.ERR_SIZE
 
 
 def test_lds_variant_limits():
@@ -75,8 +75,8 @@ def test_lds_variant_limits():
         assert L.trove_copy(fake, fake, 10, b, T.VARIANT_LDS, 64, None) == T.ERR_VARIANT
         assert L.trove_gather(fake, fake, fake, 10, b, T.VARIANT_LDS, None) == T.ERR_VARIANT
         assert L.trove_scatter(fake, fake, fake, 10, b, T.VARIANT_LDS, None) == T.ERR_VARIANT
-    assert L.trove_store_counting(fake, 10, 20, T.VARIANT_LDS, None) == T.ERR_VARIANT
-    assert L.trove_load_sum(fake, fake, 10, 20, T.VARIANT_LDS, None) == T.ERR_VARIANT
+    assert L.trove_store_counting(fake, 10, 20, T.VARIANT_LDS, 0, None) == T.ERR_VARIANT
+    assert L.trove_load_sum(fake, fake, 10, 20, T.VARIANT_LDS, 0, None) == T.ERR_VARIANT
     assert [b for b in T.STRUCT_SIZES if T.lds_supported(b)] == \
         [b for b in T.STRUCT_SIZES if b % 4 == 0 and b <= 512]
     assert all(T.word_bytes(b) == L.trove_struct_word_bytes(b) for b in T.STRUCT_SIZES)
@@ -111,13 +111,20 @@ def test_validation_before_launch():
     assert L.trove_scatter(fake, fake, fake, 10, 20, 9, None) == T.ERR_VARIANT
     assert L.trove_gather(fake, fake, fake, 10, 20, T.VARIANT_LDS + 1, None) == T.ERR_VARIANT
     assert L.trove_gather(0, 0, 0, 0, 20, 3, None) == T.OK
-    assert L.trove_store_counting(fake, 10, 6, 1, None) == T.ERR_SIZE
-    assert L.trove_load_sum(fake, fake + 1, 10, 20, 1, None) == T.ERR_ALIGN
+    assert L.trove_store_counting(fake, 10, 6, 1, 0, None) == T.ERR_SIZE
+    assert L.trove_load_sum(fake, fake + 1, 10, 20, 1, 0, None) == T.ERR_ALIGN
+    for bad_tile in (1, 3, 48, 128, -64):  # the store / load benchmarks' tile widths
+        assert L.trove_store_counting(fake, 10, 20, 1, bad_tile, None) == T.ERR_VARIANT
+        assert L.trove_load_sum(fake, fake, 10, 20, 1, bad_tile, None) == T.ERR_VARIANT
     assert L.trove_block_copy(fake, fake, 10, 17, 0, 0, None) == T.ERR_INVALID
     assert L.trove_block_copy(fake, fake + 4, 10, 4, 0, 0, None) == T.ERR_ALIGN
     assert L.trove_block_write(fake, 10, 0, None) == T.ERR_INVALID
     assert L.trove_memcpy_ceiling(fake, fake, 24, None) == T.ERR_INVALID
     assert L.trove_random_lines_ceiling(fake, 0, fake, fake, 10, None) == T.ERR_INVALID
+    assert L.trove_random_rows_write_ceiling(fake, 100, 48, fake, 10, 0, None) == T.ERR_SIZE
+    assert L.trove_random_rows_write_ceiling(fake, 0, 64, fake, 10, 0, None) == T.ERR_INVALID
+    assert L.trove_random_rows_write_ceiling(fake + 4, 100, 64, fake, 10, 0, None) == T.ERR_ALIGN
+    assert L.trove_random_rows_write_ceiling(fake + 4, 100, 60, fake + 2, 10, 1, None) == T.ERR_ALIGN
     assert L.trove_random_lines_ceiling(fake, 1 << 32, fake, fake, 10, None) == T.ERR_INVALID
     assert L.trove_random_lines_ceiling(fake + 64, 8, fake, fake, 10, None) == T.ERR_ALIGN
     assert L.trove_random_lines_ceiling(fake, 8, fake, fake, -1, None) == T.ERR_INVALID

@@ -113,6 +113,69 @@ def test_number_theory_against_library_routines():
         assert L.oracle_floor_log2(m) == m.bit_length() - 1
 
 
+def test_inverse_from_start_and_traits_closed_forms():
+    L = oracle.lib()
+    for m in range(1, 70):
+        for a in range(0, 70):
+            for r in range(1, 8):
+                want = next((x for x in range(r, m) if (x * a) % m == 1), -1)
+                assert L.oracle_mod_inverse_from(a, m, r) == want, (a, m, r)
+            assert L.oracle_mod_inverse_from(a, m, 1) == L.oracle_mod_inverse(a, m)
+    for m in range(0, 2000):
+        pow2 = m == 0 or bin(m).count("1") == 1  # the reference's (m & (m - 1)) == 0 holds for 0 too
+        assert L.oracle_pow2_odd_bits(m) == int(pow2) + 2 * (m % 2), m
+    fn = [(5 * p + 3) % 64 for p in range(200)]
+    for x in range(64):
+        for p0 in (0, 10, 63, 64):
+            want = next(p for p in range(p0, 200) if fn[p] == x)
+            assert oracle.inverse(fn, x, p0) == want, (x, p0)
+    assert oracle.inverse(fn, 64, 0) == -1  # outside the map's range
+
+
+def _memory_case(rng, s, eb, lanes, coalesced):
+    """Offsets / strides with distinct positions across lanes (disjoint
+    segments per lane, any stride >= 1 inside), or the coalesced pattern
+    offset = lane, stride = lanes."""
+    if coalesced:
+        offset = np.arange(lanes)
+        stride = np.full(lanes, lanes)
+        span = lanes * s
+    else:
+        seg = 4 * s + 3
+        stride = rng.integers(1, 5, size=lanes)
+        lead = np.array([rng.integers(0, seg - (s - 1) * int(st)) for st in stride])
+        offset = rng.permutation(lanes) * seg + lead
+        span = lanes * seg
+    return offset.astype(np.int32), stride.astype(np.int32), span
+
+
+@pytest.mark.parametrize("s,eb", [(1, 4), (2, 4), (5, 4), (16, 4), (3, 8), (2, 16), (5, 16)])
+def test_warp_memory_closed_form(s, eb):
+    rng = np.random.default_rng(s * 100 + eb)
+    for coalesced in (True, False):
+        offset, stride, span = _memory_case(rng, s, eb, 64, coalesced)
+        src = rng.integers(0, 256, size=span * eb, dtype=np.uint8)
+        init = rng.integers(0, 256, size=span * eb, dtype=np.uint8)
+        loaded, stored, unc = oracle.warp_memory(src, eb, s, offset, stride, init)
+        el = src.reshape(span, eb)
+        want_stored = init.reshape(span, eb).copy()
+        for l in range(64):
+            pos = offset[l] + stride[l] * np.arange(s)
+            assert np.array_equal(loaded[l].reshape(s, eb), el[pos]), (l, "warp_load")
+            want_stored[pos] = el[pos]
+        assert np.array_equal(stored, want_stored.reshape(-1)), "warp_store"
+        assert np.array_equal(unc, loaded), "uncoalesced_store ignores its stride"
+    # A planted mistake: a lane whose stride is off by one must load other words.
+    # (Buffers padded by s words so the longer stride stays in bounds.)
+    offset, stride, span = _memory_case(rng, s, eb, 64, False)
+    pad = span + 64 * s
+    src = rng.integers(0, 256, size=pad * eb, dtype=np.uint8)
+    if s > 1:
+        wrong, _, _ = oracle.warp_memory(src, eb, s, offset, stride + 1, np.zeros(pad * eb, np.uint8))
+        good, _, _ = oracle.warp_memory(src, eb, s, offset, stride, np.zeros(pad * eb, np.uint8))
+        assert not np.array_equal(wrong, good)
+
+
 @pytest.mark.parametrize("m", range(1, 40))
 def test_rotate_is_left_roll(m):
     t = np.arange(100, 100 + m, dtype=np.int32)
@@ -200,3 +263,41 @@ def test_counting_and_sum_match_reference_build():
             s = R.ref_counting_sum(k, g * k, 1, words.ctypes.data_as(oracle._i32p))
             assert np.array_equal(words.view(np.uint32), mine[g * k:(g + 1) * k]), (k, g)
             assert np.uint32(s & 0xFFFFFFFF) == sums[g], (k, g)
+
+
+def test_inverse_from_start_and_traits_match_reference_build():
+    R, L = _ref(), oracle.lib()
+    inv_r = _tab(R.ref_mod_inverse_r_table, 6 * 33 * 32).reshape(6, 33, 32)
+    for r in range(1, 7):
+        for a in range(0, 33):
+            for m in range(r, 33):
+                assert L.oracle_mod_inverse_from(a, m, r) == inv_r[r - 1, a, m - 1], (a, m, r)
+    bits = _tab(R.ref_bits_table, 257)
+    for m in range(0, 257):
+        assert L.oracle_pow2_odd_bits(m) == bits[m], m
+    inv = _tab(R.ref_inverse_table, 128)
+    fn = [(5 * p + 3) % 64 for p in range(200)]
+    for x in range(64):
+        assert oracle.inverse(fn, x, 0) == inv[x], x
+        assert oracle.inverse(fn, x, 10) == inv[64 + x], x
+
+
+@pytest.mark.parametrize("s", [1, 2, 5, 16])
+def test_warp_memory_matches_reference_build(s):
+    R = _ref()
+    rng = np.random.default_rng(900 + s)
+    for coalesced in (True, False):
+        offset, stride, span = _memory_case(rng, s, 4, 64, coalesced)
+        src = rng.integers(-2**31, 2**31, size=span, dtype=np.int64).astype(np.int32)
+        init = rng.integers(-2**31, 2**31, size=span, dtype=np.int64).astype(np.int32)
+        loaded, stored, unc = oracle.warp_memory(src.view(np.uint8), 4, s, offset, stride, init.view(np.uint8))
+        ref_stored = init.copy()
+        for l in range(64):
+            rl = np.zeros(s, dtype=np.int32)
+            ru = np.zeros(s, dtype=np.int32)
+            assert R.ref_warp_memory(s, src.ctypes.data_as(oracle._i32p), int(offset[l]), int(stride[l]),
+                                     rl.ctypes.data_as(oracle._i32p), ref_stored.ctypes.data_as(oracle._i32p),
+                                     ru.ctypes.data_as(oracle._i32p)) == 0
+            assert np.array_equal(loaded[l].view(np.int32), rl), (s, l)
+            assert np.array_equal(unc[l].view(np.int32), ru), (s, l)
+        assert np.array_equal(stored.view(np.int32), ref_stored), s

@@ -101,12 +101,13 @@ _SIGNATURES = {
     "trove_copy": ([_vp, _vp, _i64, _int, _int, _int, _vp], _int),
     "trove_gather": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
     "trove_scatter": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
-    "trove_store_counting": ([_vp, _i64, _int, _int, _vp], _int),
-    "trove_load_sum": ([_vp, _vp, _i64, _int, _int, _vp], _int),
+    "trove_store_counting": ([_vp, _i64, _int, _int, _int, _vp], _int),
+    "trove_load_sum": ([_vp, _vp, _i64, _int, _int, _int, _vp], _int),
     "trove_block_copy": ([_vp, _vp, _i64, _int, _int, _u64, _vp], _int),
     "trove_block_write": ([_vp, _i64, _int, _vp], _int),
     "trove_memcpy_ceiling": ([_vp, _vp, _i64, _vp], _int),
     "trove_random_lines_ceiling": ([_vp, _i64, _vp, _vp, _i64, _vp], _int),
+    "trove_random_rows_write_ceiling": ([_vp, _i64, _int, _vp, _i64, _int, _vp], _int),
     "trove_gen_words": ([_vp, _i64, _u64], _int),
     "trove_gen_permutation": ([_vp, _i64, _u64], _int),
     "trove_fill_words_device": ([_vp, _i64, _u64, _vp], _int),
@@ -209,14 +210,16 @@ def scatter(src, idx, dst, n: int, struct_bytes: int, variant="tile", stream=Non
                                _stream(stream)), "trove_scatter")
 
 
-def store_counting(dst, n: int, struct_bytes: int, variant="shfl", stream=None) -> None:
-    _check(lib().trove_store_counting(_addr(dst), n, struct_bytes, _variant(variant), _stream(stream)),
+def store_counting(dst, n: int, struct_bytes: int, variant="shfl", tile: int = 64, stream=None) -> None:
+    """dst[i] = counting_array(i * k) through store_warp_contiguous<tile> (or naive stores)."""
+    _check(lib().trove_store_counting(_addr(dst), n, struct_bytes, _variant(variant), tile, _stream(stream)),
            "trove_store_counting")
 
 
-def load_sum(src, out, n: int, struct_bytes: int, variant="shfl", stream=None) -> None:
-    _check(lib().trove_load_sum(_addr(src), _addr(out), n, struct_bytes, _variant(variant), _stream(stream)),
-           "trove_load_sum")
+def load_sum(src, out, n: int, struct_bytes: int, variant="shfl", tile: int = 64, stream=None) -> None:
+    """out[i] = wrapping sum of the words of src[i], loaded through load_warp_contiguous<tile>."""
+    _check(lib().trove_load_sum(_addr(src), _addr(out), n, struct_bytes, _variant(variant), tile,
+                                _stream(stream)), "trove_load_sum")
 
 
 def block_copy(src, dst, n_elems: int, s: int, divergent: bool = False, seed: int = 0, stream=None) -> None:
@@ -236,6 +239,15 @@ def random_lines_ceiling(buf, n_lines: int, line_idx, out, n_reads: int, stream=
     """out[r] = XOR of the 32 words of 128-byte line min(line_idx[r], n_lines - 1) of buf."""
     _check(lib().trove_random_lines_ceiling(_addr(buf), n_lines, _addr(line_idx), _addr(out), n_reads,
                                             _stream(stream)), "trove_random_lines_ceiling")
+
+
+def random_rows_write_ceiling(buf, n_rows: int, row_bytes: int, row_idx, n_writes: int, nontemporal=False,
+                              stream=None) -> None:
+    """Row min(row_idx[r], n_rows - 1) of buf (row_bytes in 32, 60, 64, 128)
+    receives the words r * (row_bytes / 4) + j, for r < n_writes."""
+    _check(lib().trove_random_rows_write_ceiling(_addr(buf), n_rows, row_bytes, _addr(row_idx), n_writes,
+                                                 int(bool(nontemporal)), _stream(stream)),
+           "trove_random_rows_write_ceiling")
 
 
 def fill_words_device(out, n: int, seed: int, stream=None) -> None:
@@ -349,14 +361,38 @@ class Server:
                  slice_indices: int = 1 << 16, variant="tile", stream=None):
         h = ctypes.c_void_p()
         self.row_bytes = row_bytes
+        self._held = {}  # request id -> device index tensor, until released
         _check(lib().trove_serve_create(_addr(table), n_rows, row_bytes, budget_bytes, max_slots, slice_indices,
                                         _variant(variant), _stream(stream), ctypes.byref(h)), "trove_serve_create")
         self._h = h
 
     def submit(self, idx, n: int, on_device: bool) -> int:
+        """Queues a gather of rows idx[0..n). Host indices: a C-contiguous
+        int32 numpy array (copied by the library before this returns).
+        Device indices: a contiguous int32 torch tensor on the GPU, kept
+        referenced here until the request is released."""
+        if n < 0:
+            raise ValueError("n must be >= 0")
+        if on_device:
+            import torch
+
+            if not isinstance(idx, torch.Tensor) or not idx.is_cuda:
+                raise ValueError("device indices must be a torch tensor on the GPU")
+            if idx.dtype != torch.int32 or not idx.is_contiguous() or idx.numel() < n:
+                raise ValueError(f"device indices must be contiguous int32 with at least {n} elements")
+            ptr = idx.data_ptr()
+        else:
+            import numpy as np
+
+            if not isinstance(idx, np.ndarray) or idx.dtype != np.int32 or not idx.flags.c_contiguous:
+                raise ValueError("host indices must be a C-contiguous int32 numpy array")
+            if idx.size < n:
+                raise ValueError(f"host indices hold {idx.size} elements, fewer than n = {n}")
+            ptr = idx.ctypes.data
         rid = ctypes.c_uint64()
-        ptr = _addr(idx) if on_device else idx.ctypes.data
         _check(lib().trove_serve_submit(self._h, ptr, n, int(on_device), ctypes.byref(rid)), "trove_serve_submit")
+        if on_device:
+            self._held[rid.value] = idx
         return rid.value
 
     def step(self) -> int:
@@ -365,6 +401,8 @@ class Server:
         return inflight.value
 
     def poll(self, rid: int):
+        """(done, latency_s); raises TroveError(ERR_RANGE) for a request
+        rejected for a device index outside the table."""
         done, lat = ctypes.c_int32(), ctypes.c_double()
         _check(lib().trove_serve_poll(self._h, rid, ctypes.byref(done), ctypes.byref(lat)), "trove_serve_poll")
         return bool(done.value), lat.value
@@ -384,6 +422,7 @@ class Server:
 
     def release(self, rid: int) -> None:
         _check(lib().trove_serve_release(self._h, rid), "trove_serve_release")
+        self._held.pop(rid, None)
 
     def drain(self) -> None:
         _check(lib().trove_serve_drain(self._h), "trove_serve_drain")
@@ -414,6 +453,7 @@ class Server:
         if self._h:
             lib().trove_serve_destroy(self._h)
             self._h = None
+            self._held.clear()
 
     def __del__(self):
         try:

@@ -76,7 +76,8 @@ __device__ __forceinline__ void copy_range_lds(const T* __restrict__ src, T* __r
 constexpr int kBatchStageBytes = 64 * 64;
 
 template <int B>
-__device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local, uint32_t* stage) {
+__device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local, uint32_t* stage, int32_t* flag) {
+    const row_guard guard{J0.last_row, flag};
     typedef blob_t<B> T;
     job_desc J = J0;
     J.src = assume_global(J.src);
@@ -96,27 +97,28 @@ __device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local, ui
         return;
     }
     if (J.variant == TROVE_VARIANT_LDS) {
-        gather_range_lds<T>(src, J.idx, dst, J.n, g0, g1, stage);
+        gather_range_lds<T>(src, J.idx, dst, J.n, g0, g1, stage, guard);
         return;
     }
     if (J.variant == TROVE_VARIANT_TILE) {  // every batch size is a multiple of 4 bytes
-        gather_range_tile<T, gather_groups_in_flight<T>()>(src, J.idx, dst, J.n, g0, g1);
+        gather_range_tile<T, gather_groups_in_flight<T>()>(src, J.idx, dst, J.n, g0, g1, guard);
         return;
     }
     if constexpr (trove::detail::use_shfl<T, 64>::value) {
         if (J.variant == TROVE_VARIANT_SHFL) {  // items of one group (trove_batch_create)
-            gather_group_shfl<T>(src, J.idx, dst, J.n, g0);
+            gather_group_shfl<T>(src, J.idx, dst, J.n, g0, guard);
             return;
         }
     }
-    gather_range_direct<T>(src, J.idx, dst, J.n, g0, g1);
+    gather_range_direct<T>(src, J.idx, dst, J.n, g0, g1, guard);
 }
 
 // Work item `it` of a job table (device memory or kernel arguments): its job
 // by binary search on first_item (wave-uniform, scalar loads), then the
 // job's path for its struct size.
 template <typename Table>
-__device__ __forceinline__ void run_item(const Table& jobs, int n_jobs, int64_t it, uint32_t* stage) {
+__device__ __forceinline__ void run_item(const Table& jobs, int n_jobs, int64_t it, uint32_t* stage,
+                                         int32_t* flags) {
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -128,10 +130,11 @@ __device__ __forceinline__ void run_item(const Table& jobs, int n_jobs, int64_t 
     }
     const job_desc J = jobs[lo];
     const int64_t local = it - J.first_item;
+    int32_t* flag = flags ? flags + lo : nullptr;
     switch (J.bytes) {
-#define TROVE_BATCH_CASE(B)      \
-    case B:                      \
-        batch_item<B>(J, local, stage); \
+#define TROVE_BATCH_CASE(B)                   \
+    case B:                                   \
+        batch_item<B>(J, local, stage, flag); \
         break;
         TROVE_BATCH_SIZES(TROVE_BATCH_CASE)
 #undef TROVE_BATCH_CASE
@@ -151,22 +154,23 @@ __device__ __forceinline__ uint32_t* batch_stage() {
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_gather(const job_desc* __restrict__ jobs, int n_jobs,
-                                                         int64_t total_items) {
+                                                         int64_t total_items, int32_t* flags) {
     const int64_t it = wave_index();
-    if (it < total_items) run_item(jobs, n_jobs, it, batch_stage());
+    if (it < total_items) run_item(jobs, n_jobs, it, batch_stage(), flags);
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_gather_pack(const job_pack p) {
     const int64_t it = wave_index();
-    if (it < p.total_items) run_item(p.jobs, p.n_jobs, it, batch_stage());
+    if (it < p.total_items) run_item(p.jobs, p.n_jobs, it, batch_stage(), p.flags);
 }
 
-int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, bool lds, hipStream_t s) {
+int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, bool lds, int32_t* flags,
+                     hipStream_t s) {
     // One wavefront per item: never capped.
     const int64_t blocks = (total_items + kWavesPerBlock - 1) / kWavesPerBlock;
     const size_t stage = lds ? (size_t)kBatchStageBytes * kWavesPerBlock : 0;
     hipLaunchKernelGGL(k_batch_gather, dim3((unsigned)blocks), dim3(kBlock), stage, s, d_table, n_jobs,
-                       total_items);
+                       total_items, flags);
     return launch_status();
 }
 
@@ -195,7 +199,32 @@ struct trove_batch {
 };
 
 static int batch_launch(trove_batch* b, hipStream_t s) {
-    return launch_job_table(b->d_table, (int)b->table.size(), b->total_items, b->lds, s);
+    return launch_job_table(b->d_table, (int)b->table.size(), b->total_items, b->lds, nullptr, s);
+}
+
+// Captures the persistent launch into b->graph / b->graph_exec once. On any
+// failure the capture stream and graph are destroyed again, so that a later
+// call starts clean.
+static int capture_graph(trove_batch* b) {
+    int rc = hip_status(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+    if (rc == TROVE_OK) {
+        rc = hip_status(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+        if (rc == TROVE_OK) {
+            const int lrc = batch_launch(b, b->capture_stream);
+            rc = hip_status(hipStreamEndCapture(b->capture_stream, &b->graph));
+            if (rc == TROVE_OK) rc = lrc;
+        }
+    }
+    if (rc == TROVE_OK) rc = hip_status(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    if (rc != TROVE_OK) {
+        if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->graph_exec = nullptr;
+        b->graph = nullptr;
+    }
+    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    b->capture_stream = nullptr;
+    return rc;
 }
 
 extern "C" {
@@ -214,7 +243,7 @@ int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out)
             delete b;
             return TROVE_ERR_SIZE;
         }
-        if (J.n < 0 || (J.n > 0 && (!J.src || !J.dst))) {
+        if (J.n < 0 || (J.n > 0 && (!J.src || !J.dst)) || J.reserved != 0) {
             delete b;
             return TROVE_ERR_INVALID;
         }
@@ -233,6 +262,7 @@ int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out)
         d.dst = J.dst;
         d.n = J.n;
         d.bytes = bytes;
+        d.last_row = 0xffffffffu;  // batch indices are the caller's to keep in range
         const bool copy = J.idx == nullptr;
         if (copy) {
             // A copy job: SHFL (load/store_warp_contiguous), LDS
@@ -276,15 +306,7 @@ int trove_batch_run(trove_batch* b, int mode, void* stream) {
     if (mode == TROVE_BATCH_PERSISTENT) return batch_launch(b, s);
     if (mode == TROVE_BATCH_GRAPH) {
         if (!b->graph_exec) {
-            int rc = hip_status(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
-            if (rc != TROVE_OK) return rc;
-            rc = hip_status(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-            if (rc != TROVE_OK) return rc;
-            const int lrc = batch_launch(b, b->capture_stream);
-            rc = hip_status(hipStreamEndCapture(b->capture_stream, &b->graph));
-            if (lrc != TROVE_OK) return lrc;
-            if (rc != TROVE_OK) return rc;
-            rc = hip_status(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+            const int rc = capture_graph(b);
             if (rc != TROVE_OK) return rc;
         }
         return hip_status(hipGraphLaunch(b->graph_exec, s));
@@ -312,7 +334,6 @@ int trove_batch_destroy(trove_batch* b) {
     if (!b) return TROVE_ERR_INVALID;
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     if (b->d_table) (void)hipFree(b->d_table);
     delete b;
     return TROVE_OK;

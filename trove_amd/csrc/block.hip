@@ -1,6 +1,8 @@
 // Block interface kernels (tests/block.cu:13-54) and plain device helpers.
 #include "common.h"
 
+#include <type_traits>
+
 namespace trove_amd {
 
 // Lanes of one 64-array group that a seeded hash sends down the fallback
@@ -188,6 +190,88 @@ int random_lines_launch(const void* buf, int64_t n_lines, const uint32_t* line_i
     const int64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
     hipLaunchKernelGGL(k_random_lines, dim3((unsigned)blocks), dim3(kBlock), 0, s, static_cast<const uint4*>(buf),
                        (uint32_t)n_lines, line_idx, out, n);
+    return launch_status();
+}
+
+// Random-row write ceiling: write r (r < n) stores ROW bytes, the words
+// w_j = r * (ROW / 4) + j, at row min(row_idx[r], n_rows - 1) of buf. A row
+// is G lanes of one word each (16-byte words when ROW is a multiple of 16,
+// else 4-byte words: 60-byte rows are 15 lanes), 64 / G rows per wavefront
+// per round, kRowRounds rounds per wavefront with every index load issued
+// before any store. The rate at which HBM absorbs random row writes of that
+// size bounds every scatter of rows of that size.
+constexpr int kRowRounds = 8;
+
+template <int ROW, bool NT>
+__global__ __launch_bounds__(kBlock) void k_random_rows_write(uint8_t* __restrict__ buf, uint32_t n_rows,
+                                                              const int32_t* __restrict__ row_idx, int64_t n) {
+    constexpr int W = ROW % 16 == 0 ? 16 : 4;  // bytes per lane
+    constexpr int G = ROW / W;                  // lanes per row
+    constexpr int P = 64 / G;                   // rows per wavefront round
+    typedef uint32_t vec4 __attribute__((ext_vector_type(4)));
+    typedef typename std::conditional<W == 16, vec4, uint32_t>::type word;
+    const int lane = trove::lane_id();
+    const int slot = lane / G, part = lane % G;
+    if (slot >= P) return;  // lanes past the last whole row (60-byte rows: lanes 60..63)
+    const int64_t r0 = wave_index() * (int64_t)(kRowRounds * P) + slot;
+    int64_t row[kRowRounds];
+#pragma unroll
+    for (int k = 0; k < kRowRounds; ++k) {
+        const int64_t r = r0 + (int64_t)P * k;
+        const uint32_t ix = (uint32_t)row_idx[r < n ? r : n - 1];
+        row[k] = ix < n_rows ? ix : n_rows - 1;
+    }
+#pragma unroll
+    for (int k = 0; k < kRowRounds; ++k) {
+        const int64_t r = r0 + (int64_t)P * k;
+        if (r < n) {
+            const uint32_t w0 = (uint32_t)(r * (ROW / 4)) + (uint32_t)(part * (W / 4));
+            word v;
+            if constexpr (W == 16) {
+                v = vec4{w0, w0 + 1, w0 + 2, w0 + 3};
+            } else {
+                v = w0;
+            }
+            word* p = reinterpret_cast<word*>(buf + row[k] * ROW + part * W);
+            if constexpr (NT) {
+                __builtin_nontemporal_store(v, p);
+            } else {
+                *p = v;
+            }
+        }
+    }
+}
+
+template <int ROW>
+static void launch_rows_write(bool nt, unsigned blocks, uint8_t* b, uint32_t n_rows, const int32_t* row_idx,
+                              int64_t n, hipStream_t s) {
+    if (nt) {
+        hipLaunchKernelGGL((k_random_rows_write<ROW, true>), dim3(blocks), dim3(kBlock), 0, s, b, n_rows, row_idx, n);
+    } else {
+        hipLaunchKernelGGL((k_random_rows_write<ROW, false>), dim3(blocks), dim3(kBlock), 0, s, b, n_rows, row_idx, n);
+    }
+}
+
+int random_rows_write_launch(void* buf, int64_t n_rows, int row_bytes, const int32_t* row_idx, int64_t n,
+                             bool nontemporal, hipStream_t s) {
+    int rows_per_wave = 0;
+    switch (row_bytes) {
+        case 32: rows_per_wave = 64 / 2; break;
+        case 60: rows_per_wave = 64 / 15; break;
+        case 64: rows_per_wave = 64 / 4; break;
+        case 128: rows_per_wave = 64 / 8; break;
+        default: return TROVE_ERR_SIZE;
+    }
+    const int64_t waves = (n + kRowRounds * rows_per_wave - 1) / (kRowRounds * rows_per_wave);
+    const unsigned blocks = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    uint8_t* b = static_cast<uint8_t*>(buf);
+    const uint32_t nr = (uint32_t)n_rows;
+    switch (row_bytes) {
+        case 32: launch_rows_write<32>(nontemporal, blocks, b, nr, row_idx, n, s); break;
+        case 60: launch_rows_write<60>(nontemporal, blocks, b, nr, row_idx, n, s); break;
+        case 64: launch_rows_write<64>(nontemporal, blocks, b, nr, row_idx, n, s); break;
+        default: launch_rows_write<128>(nontemporal, blocks, b, nr, row_idx, n, s); break;
+    }
     return launch_status();
 }
 

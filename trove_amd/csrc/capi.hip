@@ -12,6 +12,8 @@ int block_copy_dispatch(const uint32_t* x, uint32_t* r, int64_t e, int s, int di
                         hipStream_t st);
 int block_write_dispatch(uint32_t* r, int64_t l, int s, hipStream_t st);
 int memcpy_ceiling_launch(const void* src, void* dst, int64_t bytes, hipStream_t s);
+int random_rows_write_launch(void* buf, int64_t n_rows, int row_bytes, const int32_t* row_idx, int64_t n,
+                             bool nontemporal, hipStream_t s);
 int random_lines_launch(const void* buf, int64_t n_lines, const uint32_t* line_idx, uint32_t* out, int64_t n,
                         hipStream_t s);
 int fill_words_launch(uint32_t* out, int64_t n, uint64_t seed, hipStream_t s);
@@ -83,8 +85,8 @@ int64_t grid_for(int64_t waves, int waves_per_block) {
 
 typedef int (*copy_fn)(const void*, void*, int64_t, int, int, hipStream_t);
 typedef int (*index_fn)(const void*, const int32_t*, void*, int64_t, int, hipStream_t);
-typedef int (*counting_fn)(void*, int64_t, int, hipStream_t);
-typedef int (*loadsum_fn)(const void*, uint32_t*, int64_t, int, hipStream_t);
+typedef int (*counting_fn)(void*, int64_t, int, int, hipStream_t);
+typedef int (*loadsum_fn)(const void*, uint32_t*, int64_t, int, int, hipStream_t);
 
 struct size_entry {
     int bytes;
@@ -107,6 +109,8 @@ static const size_entry* find_size(int bytes) {
 }
 
 static bool valid_variant(int v) { return v >= TROVE_VARIANT_AUTO && v <= TROVE_VARIANT_LDS; }
+// Tile widths of the warp-contiguous API: 0 (= 64) or a power of two 2..64.
+static bool valid_tile(int tile) { return tile == 0 || (tile >= 2 && tile <= 64 && (tile & (tile - 1)) == 0); }
 
 // Largest n the one-unit-per-wavefront kernels cover with a grid of at most
 // 2^31 - 1 single-wavefront work-groups (2^37 structs, far beyond any HBM).
@@ -158,7 +162,7 @@ int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int vari
     if (!e) return TROVE_ERR_SIZE;
     if (n < 0 || n > kMaxStructs) return TROVE_ERR_INVALID;
     if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE) return TROVE_ERR_VARIANT;
-    if (tile != 0 && (tile < 2 || tile > 64 || (tile & (tile - 1)) != 0)) return TROVE_ERR_VARIANT;
+    if (!valid_tile(tile)) return TROVE_ERR_VARIANT;
     if (variant == TROVE_VARIANT_LDS && (tile != 0 && tile != 64)) return TROVE_ERR_VARIANT;  // whole wavefronts
     if (n == 0) return TROVE_OK;
     if (!src || !dst) return TROVE_ERR_INVALID;
@@ -201,28 +205,29 @@ int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int
     return e->scatter(src, idx, dst, n, variant, static_cast<hipStream_t>(stream));
 }
 
-int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, void* stream) {
+int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, int tile, void* stream) {
     const size_entry* e = find_size(struct_bytes);
     if (!e || struct_bytes % 4 != 0) return TROVE_ERR_SIZE;
     if (n < 0) return TROVE_ERR_INVALID;
-    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS)
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS || !valid_tile(tile))
         return TROVE_ERR_VARIANT;
     if (n == 0) return TROVE_OK;
     if (!dst) return TROVE_ERR_INVALID;
     if (!aligned(dst, word_bytes_of(struct_bytes))) return TROVE_ERR_ALIGN;
-    return e->counting(dst, n, variant, static_cast<hipStream_t>(stream));
+    return e->counting(dst, n, variant, tile, static_cast<hipStream_t>(stream));
 }
 
-int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, int variant, void* stream) {
+int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, int variant, int tile,
+                   void* stream) {
     const size_entry* e = find_size(struct_bytes);
     if (!e || struct_bytes % 4 != 0) return TROVE_ERR_SIZE;
     if (n < 0) return TROVE_ERR_INVALID;
-    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS)
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS || !valid_tile(tile))
         return TROVE_ERR_VARIANT;
     if (n == 0) return TROVE_OK;
     if (!src || !out) return TROVE_ERR_INVALID;
     if (!aligned(src, word_bytes_of(struct_bytes)) || !aligned(out, 4)) return TROVE_ERR_ALIGN;
-    return e->loadsum(src, out, n, variant, static_cast<hipStream_t>(stream));
+    return e->loadsum(src, out, n, variant, tile, static_cast<hipStream_t>(stream));
 }
 
 int trove_block_copy(const void* src, void* dst, int64_t n_elems, int s, int divergent, uint64_t seed,
@@ -259,6 +264,17 @@ int trove_random_lines_ceiling(const void* buf, int64_t n_lines, const uint32_t*
     if (!buf || !line_idx || !out) return TROVE_ERR_INVALID;
     if (!aligned(buf, 128) || !aligned(line_idx, 4) || !aligned(out, 4)) return TROVE_ERR_ALIGN;
     return random_lines_launch(buf, n_lines, line_idx, out, n_reads, static_cast<hipStream_t>(stream));
+}
+
+int trove_random_rows_write_ceiling(void* buf, int64_t n_rows, int row_bytes, const int32_t* row_idx,
+                                    int64_t n_writes, int nontemporal, void* stream) {
+    if (row_bytes != 32 && row_bytes != 60 && row_bytes != 64 && row_bytes != 128) return TROVE_ERR_SIZE;
+    if (n_rows <= 0 || n_rows > 0xffffffffLL || n_writes < 0 || n_writes > kMaxStructs) return TROVE_ERR_INVALID;
+    if (n_writes == 0) return TROVE_OK;
+    if (!buf || !row_idx) return TROVE_ERR_INVALID;
+    if (!aligned(buf, row_bytes % 16 == 0 ? 16 : 4) || !aligned(row_idx, 4)) return TROVE_ERR_ALIGN;
+    return random_rows_write_launch(buf, n_rows, row_bytes, row_idx, n_writes, nontemporal != 0,
+                                    static_cast<hipStream_t>(stream));
 }
 
 int trove_gen_words(uint32_t* out, int64_t n, uint64_t seed) {

@@ -176,8 +176,8 @@ int gather_impl(const void* src, const int32_t* idx, void* dst, int64_t n, int v
 template <int B>
 int scatter_impl(const void* src, const int32_t* idx, void* dst, int64_t n, int variant, hipStream_t s);
 template <int B>
-int counting_impl(void* dst, int64_t n, int variant, hipStream_t s);
+int counting_impl(void* dst, int64_t n, int variant, int tile, hipStream_t s);
 template <int B>
-int loadsum_impl(const void* src, uint32_t* out, int64_t n, int variant, hipStream_t s);
+int loadsum_impl(const void* src, uint32_t* out, int64_t n, int variant, int tile, hipStream_t s);
 
 }  // namespace trove_amd

@@ -227,7 +227,9 @@ __device__ __forceinline__ int64_t bench_wave() {
     }
 }
 
-template <typename T, int UNROLL, int B, bool XCD>
+// TILE: the store_warp_contiguous / load_warp_contiguous tile width (the
+// reference benchmark's TILE_SIZE, tests/Makefile:4; 64 = a whole wavefront).
+template <typename T, int TILE, int UNROLL, int B, bool XCD>
 __global__ __launch_bounds__(B) void k_counting_shfl(T* __restrict__ dst, int64_t n) {
     const int lane = trove::lane_id();
     const int64_t groups = n >> 6;
@@ -237,12 +239,12 @@ __global__ __launch_bounds__(B) void k_counting_shfl(T* __restrict__ dst, int64_
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             const int64_t i = ((g + u) << 6) + lane;
-            trove::store_warp_contiguous<trove::WARP_SIZE, trove::streaming>(counting_struct<T>(i), dst + i);
+            trove::store_warp_contiguous<TILE, trove::streaming>(counting_struct<T>(i), dst + i);
         }
     }
     for (; g < groups; ++g) {
         const int64_t i = (g << 6) + lane;
-        trove::store_warp_contiguous<trove::WARP_SIZE, trove::streaming>(counting_struct<T>(i), dst + i);
+        trove::store_warp_contiguous<TILE, trove::streaming>(counting_struct<T>(i), dst + i);
     }
     if (wave_index<B>() == 0 && (n & 63)) {
         const int64_t i = (groups << 6) + lane;
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_counting_direct(T* __restrict__ dst,
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = counting_struct<T>(i);
 }
 
-template <typename T, int UNROLL, int B, bool XCD>
+template <typename T, int TILE, int UNROLL, int B, bool XCD>
 __global__ __launch_bounds__(B) void k_loadsum_shfl(const T* __restrict__ src, uint32_t* __restrict__ out,
                                                     int64_t n) {
     const int lane = trove::lane_id();
@@ -267,13 +269,13 @@ __global__ __launch_bounds__(B) void k_loadsum_shfl(const T* __restrict__ src, u
         T v[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u)
-            v[u] = trove::load_warp_contiguous<trove::WARP_SIZE, trove::streaming>(src + ((g + u) << 6) + lane);
+            v[u] = trove::load_warp_contiguous<TILE, trove::streaming>(src + ((g + u) << 6) + lane);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) out[((g + u) << 6) + lane] = trove::sum(v[u]);
     }
     for (; g < groups; ++g) {
         const int64_t i = (g << 6) + lane;
-        out[i] = trove::sum(trove::load_warp_contiguous<trove::WARP_SIZE, trove::streaming>(src + i));
+        out[i] = trove::sum(trove::load_warp_contiguous<TILE, trove::streaming>(src + i));
     }
     if (wave_index<B>() == 0 && (n & 63)) {
         const int64_t i = (groups << 6) + lane;
@@ -325,22 +327,49 @@ struct bench_shape {
     static constexpr bool load_xcd = words >= 5;
 };
 
+// Launches the store-only / load-only kernel at tile width TILE with the
+// launch shape measured for the whole-wavefront tile.
+template <typename T, int TILE>
+static int launch_counting(T* dst, int64_t n, hipStream_t s) {
+    typedef copy_shape<T> C;
+    typedef bench_shape<T> H;
+    constexpr int WB = H::store_block;
+    const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
+    const int64_t blocks = grid_for(units, WB / 64);
+    hipLaunchKernelGGL((k_counting_shfl<T, TILE, C::unroll, WB, H::store_xcd>), dim3((unsigned)blocks), dim3(WB),
+                       cap_lds_reservation(H::store_cap), s, dst, n);
+    return launch_status();
+}
+
+template <typename T, int TILE>
+static int launch_loadsum(const T* src, uint32_t* out, int64_t n, hipStream_t s) {
+    typedef copy_shape<T> C;
+    typedef bench_shape<T> H;
+    constexpr int WB = H::load_block;
+    const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
+    const int64_t blocks = grid_for(units, WB / 64);
+    hipLaunchKernelGGL((k_loadsum_shfl<T, TILE, C::unroll, WB, H::load_xcd>), dim3((unsigned)blocks), dim3(WB),
+                       cap_lds_reservation(H::load_cap), s, src, out, n);
+    return launch_status();
+}
+
 template <int B>
-int counting_impl(void* dst_, int64_t n, int variant, hipStream_t s) {
+int counting_impl(void* dst_, int64_t n, int variant, int tile, hipStream_t s) {
     if constexpr (B % 4 != 0 || B > kMaxBenchStructBytes) {
         return TROVE_ERR_SIZE;
     } else {
         typedef blob_t<B> T;
-        typedef copy_shape<T> C;
         T* dst = static_cast<T*>(dst_);
         if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL) {
-            typedef bench_shape<T> H;
-            constexpr int WB = H::store_block;
-            const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
-            const int64_t blocks = grid_for(units, WB / 64);
-            hipLaunchKernelGGL((k_counting_shfl<T, C::unroll, WB, H::store_xcd>), dim3((unsigned)blocks), dim3(WB),
-                               cap_lds_reservation(H::store_cap), s, dst, n);
-            return launch_status();
+            switch (tile == 0 ? 64 : tile) {
+                case 64: return launch_counting<T, 64>(dst, n, s);
+                case 32: return launch_counting<T, 32>(dst, n, s);
+                case 16: return launch_counting<T, 16>(dst, n, s);
+                case 8: return launch_counting<T, 8>(dst, n, s);
+                case 4: return launch_counting<T, 4>(dst, n, s);
+                case 2: return launch_counting<T, 2>(dst, n, s);
+                default: return TROVE_ERR_VARIANT;
+            }
         }
         if (variant == TROVE_VARIANT_DIRECT) {
             const int64_t blocks = grid_for((n + 63) / 64);
@@ -352,21 +381,22 @@ int counting_impl(void* dst_, int64_t n, int variant, hipStream_t s) {
 }
 
 template <int B>
-int loadsum_impl(const void* src_, uint32_t* out, int64_t n, int variant, hipStream_t s) {
+int loadsum_impl(const void* src_, uint32_t* out, int64_t n, int variant, int tile, hipStream_t s) {
     if constexpr (B % 4 != 0 || B > kMaxBenchStructBytes) {
         return TROVE_ERR_SIZE;
     } else {
         typedef blob_t<B> T;
-        typedef copy_shape<T> C;
         const T* src = static_cast<const T*>(src_);
         if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL) {
-            typedef bench_shape<T> H;
-            constexpr int WB = H::load_block;
-            const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
-            const int64_t blocks = grid_for(units, WB / 64);
-            hipLaunchKernelGGL((k_loadsum_shfl<T, C::unroll, WB, H::load_xcd>), dim3((unsigned)blocks), dim3(WB),
-                               cap_lds_reservation(H::load_cap), s, src, out, n);
-            return launch_status();
+            switch (tile == 0 ? 64 : tile) {
+                case 64: return launch_loadsum<T, 64>(src, out, n, s);
+                case 32: return launch_loadsum<T, 32>(src, out, n, s);
+                case 16: return launch_loadsum<T, 16>(src, out, n, s);
+                case 8: return launch_loadsum<T, 8>(src, out, n, s);
+                case 4: return launch_loadsum<T, 4>(src, out, n, s);
+                case 2: return launch_loadsum<T, 2>(src, out, n, s);
+                default: return TROVE_ERR_VARIANT;
+            }
         }
         if (variant == TROVE_VARIANT_DIRECT) {
             const int64_t blocks = grid_for((n + 63) / 64);
@@ -379,8 +409,8 @@ int loadsum_impl(const void* src_, uint32_t* out, int64_t n, int variant, hipStr
 
 #define TROVE_INST_COPY(B)                                                          \
     template int copy_impl<B>(const void*, void*, int64_t, int, int, hipStream_t); \
-    template int counting_impl<B>(void*, int64_t, int, hipStream_t);               \
-    template int loadsum_impl<B>(const void*, uint32_t*, int64_t, int, hipStream_t);
+    template int counting_impl<B>(void*, int64_t, int, int, hipStream_t);          \
+    template int loadsum_impl<B>(const void*, uint32_t*, int64_t, int, int, hipStream_t);
 TROVE_FOR_EACH_SIZE_IN_SHARD(TROVE_INST_COPY)
 
 }  // namespace trove_amd

@@ -9,6 +9,26 @@
 
 namespace trove_amd {
 
+// Index guards: every gather body passes each index it reads through one
+// before using it as a row. no_guard trusts the caller (the C ABI's
+// gathers); row_guard maps an index outside [0, last] to row 0 and records
+// the violation in *flag (when set), so that a job with a bad index never
+// reads outside its source. The flag is written with a system-scope store so
+// that it may live in pinned host memory.
+struct no_guard {
+    __device__ __forceinline__ int32_t operator()(int32_t ix) const { return ix; }
+};
+
+struct row_guard {
+    uint32_t last;  // largest valid row (0xffffffff: every index valid)
+    int32_t* flag;  // nullable
+    __device__ __forceinline__ int32_t operator()(int32_t ix) const {
+        if ((uint32_t)ix <= last) return ix;
+        if (flag) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return 0;
+    }
+};
+
 // 64-struct groups a wavefront keeps in flight on the tile path. Random rows
 // cost whole 128-byte line fills, and the HBM serves random lines faster the
 // more of them are outstanding (measured on MI355X: 3.8 TB/s of line reads
@@ -30,9 +50,10 @@ constexpr int gather_groups_in_flight() {
 // One-word structs (m = 1: 4, 8 and 16 bytes) have nothing to broadcast or
 // transpose: each lane loads its row as one word. They keep the tile path's
 // policies (streaming index loads and stores, UNROLL groups in flight).
-template <typename T, int UNROLL>
+template <typename T, int UNROLL, typename G = no_guard>
 __device__ __forceinline__ void gather_range_word(const T* __restrict__ src, const int32_t* __restrict__ idx,
-                                                  T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1) {
+                                                  T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1,
+                                                  G guard = G()) {
     typedef typename trove::detail::word_layout<T>::U U;
     typedef trove::streaming S;
     const U* s = reinterpret_cast<const U*>(src);
@@ -45,7 +66,7 @@ __device__ __forceinline__ void gather_range_word(const T* __restrict__ src, con
         int32_t ix[UNROLL];
         U v[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) ix[u] = trove::detail::load_word<S>(idx + ((g + u) << 6) + lane);
+        for (int u = 0; u < UNROLL; ++u) ix[u] = guard(trove::detail::load_word<S>(idx + ((g + u) << 6) + lane));
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) v[u] = trove::detail::load_word<trove::cached>(s + ix[u]);
 #pragma unroll
@@ -53,21 +74,22 @@ __device__ __forceinline__ void gather_range_word(const T* __restrict__ src, con
     }
     for (; g < full_end; ++g) {
         const int64_t i = (g << 6) + lane;
-        trove::detail::store_word<S>(d + i, trove::detail::load_word<trove::cached>(s + trove::detail::load_word<S>(idx + i)));
+        trove::detail::store_word<S>(d + i, trove::detail::load_word<trove::cached>(s + guard(trove::detail::load_word<S>(idx + i))));
     }
     if (g1 > full && (n & 63)) {
         const int64_t i = (full << 6) + lane;
-        if (i < n) d[i] = s[idx[i]];
+        if (i < n) d[i] = s[guard(idx[i])];
     }
 }
 
-template <typename T, int UNROLL>
+template <typename T, int UNROLL, typename G = no_guard>
 __device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
-                                                  T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1) {
+                                                  T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1,
+                                                  G guard = G()) {
     typedef trove::detail::word_layout<T> L;
     typedef trove::streaming S;
     if constexpr (L::m == 1) {
-        gather_range_word<T, UNROLL>(src, idx, dst, n, g0, g1);
+        gather_range_word<T, UNROLL>(src, idx, dst, n, g0, g1, guard);
         return;
     }
     const int lane = trove::lane_id();
@@ -77,7 +99,7 @@ __device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, con
     for (; g + UNROLL <= full_end; g += UNROLL) {
         int32_t ix[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) ix[u] = trove::detail::load_word<S>(idx + ((g + u) << 6) + lane);
+        for (int u = 0; u < UNROLL; ++u) ix[u] = guard(trove::detail::load_word<S>(idx + ((g + u) << 6) + lane));
         uint32_t v[UNROLL][L::m * L::W];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) trove::detail::tile_gather_indexed<64>(src, ix[u], lane, 0, v[u]);
@@ -86,12 +108,12 @@ __device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, con
     }
     for (; g < full_end; ++g) {
         uint32_t v[L::m * L::W];
-        trove::detail::tile_gather_indexed<64>(src, trove::detail::load_word<S>(idx + (g << 6) + lane), lane, 0, v);
+        trove::detail::tile_gather_indexed<64>(src, guard(trove::detail::load_word<S>(idx + (g << 6) + lane)), lane, 0, v);
         trove::detail::tile_store_words<64, T, S>(dst + (g << 6), lane, v);
     }
     if (g1 > full && (n & 63)) {
         const int64_t i = (full << 6) + lane;
-        if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + idx[i]), dst + i);
+        if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + guard(idx[i])), dst + i);
     }
 }
 
@@ -99,14 +121,14 @@ __device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, con
 // last group runs the tile cascade with its missing lanes inactive. There is
 // deliberately no loop here: around the cascade, LICM would keep the tables
 // of all six tile levels live at once.
-template <typename T>
+template <typename T, typename G = no_guard>
 __device__ __forceinline__ void gather_group_shfl(const T* __restrict__ src, const int32_t* __restrict__ idx,
-                                                  T* __restrict__ dst, int64_t n, int64_t g) {
+                                                  T* __restrict__ dst, int64_t n, int64_t g, G guard = G()) {
     const int64_t i = (g << 6) + trove::lane_id();
     if (i < n) {
         trove::coalesced_ptr<T> s(const_cast<T*>(src));
         trove::coalesced_ptr<T> r(dst);
-        const T v = s[idx[i]];
+        const T v = s[guard(idx[i])];
         r[i] = v;
     }
 }
@@ -117,13 +139,13 @@ __device__ __forceinline__ void gather_group_shfl(const T* __restrict__ src, con
 // delivers it, then back through the stage to CW-dword chunk stores. stage:
 // 64 * sizeof(T) bytes of LDS private to the wavefront. One-word structs
 // (m = 1) load their row per lane and only stage the store.
-template <typename T, int CW>
+template <typename T, int CW, typename G = no_guard>
 __device__ __forceinline__ void gather_group_lds(const T* __restrict__ src, const int32_t* __restrict__ idx,
-                                                 T* __restrict__ dst, int64_t g, uint32_t* stage) {
+                                                 T* __restrict__ dst, int64_t g, uint32_t* stage, G guard = G()) {
     namespace d = trove::detail;
     typedef d::word_layout<T> L;
     const int lane = trove::lane_id();
-    const int32_t ix = d::load_word<trove::streaming>(idx + (g << 6) + lane);
+    const int32_t ix = guard(d::load_word<trove::streaming>(idx + (g << 6) + lane));
     T v;
     if constexpr (L::m == 1) {
         v = d::divergent_load(src + ix);
@@ -144,31 +166,32 @@ __device__ __forceinline__ void gather_group_lds(const T* __restrict__ src, cons
 }
 
 // LDS-staged gather over groups [g0, g1); the partial last group per lane.
-template <typename T>
+template <typename T, typename G = no_guard>
 __device__ __forceinline__ void gather_range_lds(const T* __restrict__ src, const int32_t* __restrict__ idx,
                                                  T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1,
-                                                 uint32_t* stage) {
+                                                 uint32_t* stage, G guard = G()) {
     const int64_t full = n >> 6;
     const int64_t full_end = g1 < full ? g1 : full;
     if (trove::detail::lds_wide(dst)) {
-        for (int64_t g = g0; g < full_end; ++g) gather_group_lds<T, 4>(src, idx, dst, g, stage);
+        for (int64_t g = g0; g < full_end; ++g) gather_group_lds<T, 4>(src, idx, dst, g, stage, guard);
     } else {
-        for (int64_t g = g0; g < full_end; ++g) gather_group_lds<T, 1>(src, idx, dst, g, stage);
+        for (int64_t g = g0; g < full_end; ++g) gather_group_lds<T, 1>(src, idx, dst, g, stage, guard);
     }
     if (g1 > full && (n & 63)) {
         const int64_t i = (full << 6) + trove::lane_id();
-        if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + idx[i]), dst + i);
+        if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + guard(idx[i])), dst + i);
     }
 }
 
 // Naive per-lane gather (tests/benchmark.cu:112-119).
-template <typename T>
+template <typename T, typename G = no_guard>
 __device__ __forceinline__ void gather_range_direct(const T* __restrict__ src, const int32_t* __restrict__ idx,
-                                                    T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1) {
+                                                    T* __restrict__ dst, int64_t n, int64_t g0, int64_t g1,
+                                                    G guard = G()) {
     const int lane = trove::lane_id();
     for (int64_t g = g0; g < g1; ++g) {
         const int64_t i = (g << 6) + lane;
-        if (i < n) dst[i] = src[idx[i]];
+        if (i < n) dst[i] = src[guard(idx[i])];
     }
 }
 

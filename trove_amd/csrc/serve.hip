@@ -1,6 +1,7 @@
 // Continuously batched gather server over a device-resident struct table.
 // (Device-resident request indices are read in place: the caller keeps them
-// valid until the request is done, as for host indices.)
+// valid until the request is done. Host indices are range-checked and copied
+// at submission, so the caller's array may change right after.)
 //
 // No reference counterpart: the reference is a device library. This engine
 // serves requests "gather these rows of the table" with:
@@ -19,7 +20,11 @@
 //    host index uploads run on a separate copy stream and the compute stream
 //    waits on them by event;
 //  * completion: a request is done when the step holding its last slice has
-//    finished (event query); its output stays in the arena until released.
+//    finished (event query); its output stays in the arena until released;
+//  * index safety: every step's gather checks each index against the table
+//    (an index outside it reads row 0 instead) and sets the job's flag in
+//    pinned host memory; a request with a flagged slice completes as
+//    rejected (TROVE_ERR_RANGE) and its rows are never handed out.
 // The host side is single-threaded: the caller drives it with step() (or
 // replay()), and every call is non-blocking except drain().
 #include "batch.h"
@@ -81,8 +86,10 @@ enum req_state { QUEUED = 0, ACTIVE = 1, DONE = 2 };
 
 struct request {
     uint64_t id = 0;
-    const int32_t* idx = nullptr;  // caller's indices (host or device)
+    const int32_t* idx = nullptr;  // caller's device indices, or host_idx.data()
+    std::vector<int32_t> host_idx;  // copy of host indices, taken at submission
     bool idx_on_device = false;
+    bool bad_index = false;  // a step flagged an index outside the table
     int64_t n = 0;
     int64_t rows = 0;       // arena rows reserved (>= 1)
     int64_t row0 = -1;      // arena offset
@@ -95,6 +102,7 @@ struct request {
 struct step_rec {
     hipEvent_t done = nullptr;
     std::vector<uint64_t> finishing;
+    std::vector<uint64_t> job_req;  // request id of each job of the step
     bool busy = false;
 };
 
@@ -126,11 +134,14 @@ struct trove_serve {
     step_rec steps[kInflight];
     job_desc* h_jobs[kInflight] = {};  // pinned
     job_desc* d_jobs[kInflight] = {};
+    int32_t* h_flags[kInflight] = {};  // pinned, coherent: per-job bad-index flags the step kernel sets
+    int32_t* d_flags[kInflight] = {};  // the same memory as the device addresses it
     int max_jobs = 0;
     job_pack pack = {};  // kernel-argument job table of small steps
     uint64_t step_seq = 0;  // next step slot = step_seq % kInflight
     uint64_t retired_seq = 0;
-    std::vector<uint64_t> newly_done;  // completions not yet seen by replay
+    std::vector<uint64_t> newly_done;  // completions not yet seen by replay (recorded while it runs)
+    bool replaying = false;
     // statistics
     int64_t completed = 0, indices_done = 0, steps_launched = 0;
 };
@@ -139,6 +150,7 @@ static void free_serve(trove_serve* s) {
     for (int k = 0; k < kInflight; ++k) {
         if (s->steps[k].done) (void)hipEventDestroy(s->steps[k].done);
         if (s->h_jobs[k]) (void)hipHostFree(s->h_jobs[k]);
+        if (s->h_flags[k]) (void)hipHostFree(s->h_flags[k]);
         if (s->d_jobs[k]) (void)hipFree(s->d_jobs[k]);
     }
     if (s->upload_done) (void)hipEventDestroy(s->upload_done);
@@ -157,13 +169,18 @@ static int retire(trove_serve* s) {
         if (q == hipErrorNotReady) break;
         if (q != hipSuccess) return hip_status(q);
         const double t = now_s();
+        const volatile int32_t* flags = s->h_flags[s->retired_seq % kInflight];
+        for (size_t j = 0; j < st.job_req.size(); ++j)
+            if (flags[j]) s->reqs[st.job_req[j]].bad_index = true;
+        st.job_req.clear();
         for (uint64_t id : st.finishing) {
             request& r = s->reqs[id];
             r.state = DONE;
             r.t_done = t;
+            std::vector<int32_t>().swap(r.host_idx);  // uploaded and gathered: no longer needed
             s->completed++;
             s->indices_done += r.n;
-            s->newly_done.push_back(id);
+            if (s->replaying) s->newly_done.push_back(id);
         }
         st.finishing.clear();
         st.busy = false;
@@ -211,6 +228,7 @@ static int launch_step(trove_serve* s) {
     const int k = (int)(s->step_seq % kInflight);
     step_rec& st = s->steps[k];
     job_desc* jobs = s->h_jobs[k];
+    const int32_t last_row = (int32_t)(s->n_rows - 1);
     int nj = 0;
     int64_t items = 0;
     std::vector<uint64_t> still_active;
@@ -236,8 +254,9 @@ static int launch_step(trove_serve* s) {
         // saturation +3% and p50 / p99 at 80-95% load 20-30% lower; one group
         // per item costs 7% of saturation throughput.
         d.chunk_groups = s->variant == TROVE_VARIANT_SHFL ? 1 : 2;
-        d.pad = 0;
+        d.last_row = (uint32_t)last_row;
         d.first_item = items;
+        st.job_req.push_back(id);
         items += ((cnt + 63) / 64 + d.chunk_groups - 1) / d.chunk_groups;
         r.scheduled += cnt;
         if (r.scheduled == r.n)
@@ -247,6 +266,8 @@ static int launch_step(trove_serve* s) {
     }
     if (nj == 0) return TROVE_OK;
     s->active.swap(still_active);
+    // The slot's previous step has retired: its flags are free to clear.
+    for (int j = 0; j < nj; ++j) s->h_flags[k][j] = 0;
     int rc = TROVE_OK;
     if (nj <= kArgJobs) {
         // Small steps carry their jobs in the kernel arguments: no upload.
@@ -255,10 +276,12 @@ static int launch_step(trove_serve* s) {
         p.n_jobs = nj;
         p.pad = 0;
         p.total_items = items;
+        p.flags = s->d_flags[k];
         rc = launch_job_pack(p, s->variant == TROVE_VARIANT_LDS, s->stream);
     } else {
         rc = hip_status(hipMemcpyAsync(s->d_jobs[k], jobs, sizeof(job_desc) * nj, hipMemcpyHostToDevice, s->stream));
-        if (rc == TROVE_OK) rc = launch_job_table(s->d_jobs[k], nj, items, s->variant == TROVE_VARIANT_LDS, s->stream);
+        if (rc == TROVE_OK)
+            rc = launch_job_table(s->d_jobs[k], nj, items, s->variant == TROVE_VARIANT_LDS, s->d_flags[k], s->stream);
     }
     if (rc == TROVE_OK) rc = hip_status(hipEventRecord(st.done, s->stream));
     if (rc != TROVE_OK) return rc;
@@ -315,6 +338,10 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
         rc = hip_status(hipEventCreateWithFlags(&s->steps[k].done, hipEventDisableTiming));
         if (rc == TROVE_OK) rc = hip_status(hipHostMalloc((void**)&s->h_jobs[k], sizeof(job_desc) * s->max_jobs));
         if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->d_jobs[k], sizeof(job_desc) * s->max_jobs));
+        if (rc == TROVE_OK)
+            rc = hip_status(hipHostMalloc((void**)&s->h_flags[k], sizeof(int32_t) * s->max_jobs,
+                                          hipHostMallocMapped | hipHostMallocCoherent));
+        if (rc == TROVE_OK) rc = hip_status(hipHostGetDevicePointer((void**)&s->d_flags[k], s->h_flags[k], 0));
     }
     if (rc != TROVE_OK) {
         free_serve(s);
@@ -335,14 +362,21 @@ int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on
     }
     request r;
     r.id = s->next_id++;
-    r.idx = idx;
     r.idx_on_device = idx_on_device != 0;
+    if (r.idx_on_device) {
+        r.idx = idx;
+    } else if (n > 0) {
+        // Copied right after the check: later changes to the caller's array
+        // (or its release) cannot reach the gather.
+        r.host_idx.assign(idx, idx + n);
+    }
     r.n = n;
     r.rows = n > 0 ? n : 1;
     r.t_submit = now_s();
     *request_id = r.id;
     auto& slot = s->reqs[r.id];
-    slot = r;
+    slot = std::move(r);
+    if (!slot.idx_on_device) slot.idx = slot.host_idx.data();
     if (n == 0) {
         finish_empty(s, slot);
         slot.row0 = -1;
@@ -365,7 +399,7 @@ int trove_serve_poll(trove_serve* s, uint64_t request_id, int32_t* done, double*
     if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
     *done = it->second.state == DONE;
     if (latency_s) *latency_s = it->second.state == DONE ? it->second.t_done - it->second.t_submit : -1.0;
-    return TROVE_OK;
+    return it->second.state == DONE && it->second.bad_index ? TROVE_ERR_RANGE : TROVE_OK;
 }
 
 int trove_serve_output(trove_serve* s, uint64_t request_id, void** dev_out) {
@@ -373,6 +407,7 @@ int trove_serve_output(trove_serve* s, uint64_t request_id, void** dev_out) {
     auto it = s->reqs.find(request_id);
     if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
     if (it->second.state != DONE) return TROVE_ERR_AGAIN;
+    if (it->second.bad_index) return TROVE_ERR_RANGE;
     *dev_out = it->second.row0 >= 0 ? s->out_arena + it->second.row0 * (int64_t)s->row_bytes : nullptr;
     return TROVE_OK;
 }
@@ -382,6 +417,7 @@ int trove_serve_fetch(trove_serve* s, uint64_t request_id, void* out) {
     auto it = s->reqs.find(request_id);
     if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
     if (it->second.state != DONE) return TROVE_ERR_AGAIN;
+    if (it->second.bad_index) return TROVE_ERR_RANGE;
     if (it->second.n == 0) return TROVE_OK;
     const void* src = s->out_arena + it->second.row0 * (int64_t)s->row_bytes;
     return hip_status(hipMemcpy(out, src, (size_t)(it->second.n * s->row_bytes), hipMemcpyDefault));
@@ -425,14 +461,16 @@ int trove_serve_stats(const trove_serve* s, int64_t* completed, int64_t* indices
     return TROVE_OK;
 }
 
-int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
+// The replay loop proper; trove_serve_replay brackets it with the recording
+// of completions in newly_done.
+static int replay_loop(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
                        const int64_t* n, int idx_on_device, int release_on_done, double* latency_s) {
-    if (!s || n_req < 0 || (n_req > 0 && (!arrival_s || !idx || !n || !latency_s))) return TROVE_ERR_INVALID;
     std::vector<uint64_t> ids(n_req, 0);
     std::unordered_map<uint64_t, int32_t> slot_of;
     slot_of.reserve((size_t)n_req * 2);
     int32_t next = 0, done = 0;
-    s->newly_done.clear();
+    // A rejected request (an index outside the table) reports latency -1.
+    auto latency_of = [](const request& r) { return r.bad_index ? -1.0 : r.t_done - r.t_submit; };
     const double t0 = now_s();
     while (done < n_req) {
         const double t = now_s() - t0;
@@ -443,7 +481,7 @@ int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, c
             request& r = s->reqs[ids[next]];
             r.t_submit = t0 + arrival_s[next];
             if (r.state == DONE) {
-                latency_s[next] = r.t_done - r.t_submit;
+                latency_s[next] = latency_of(r);
                 ++done;
                 if (release_on_done) trove_serve_release(s, ids[next]);
             } else {
@@ -461,8 +499,7 @@ int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, c
         for (uint64_t id : s->newly_done) {
             auto it = slot_of.find(id);
             if (it == slot_of.end()) continue;
-            const request& r = s->reqs[id];
-            latency_s[it->second] = r.t_done - r.t_submit;
+            latency_s[it->second] = latency_of(s->reqs[id]);
             ++done;
             slot_of.erase(it);
             if (release_on_done) trove_serve_release(s, id);
@@ -470,6 +507,17 @@ int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, c
         s->newly_done.clear();
     }
     return TROVE_OK;
+}
+
+int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
+                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s) {
+    if (!s || n_req < 0 || (n_req > 0 && (!arrival_s || !idx || !n || !latency_s))) return TROVE_ERR_INVALID;
+    s->newly_done.clear();
+    s->replaying = true;
+    const int rc = replay_loop(s, n_req, arrival_s, idx, n, idx_on_device, release_on_done, latency_s);
+    s->replaying = false;
+    s->newly_done.clear();
+    return rc;
 }
 
 int trove_serve_destroy(trove_serve* s) {
