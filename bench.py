@@ -400,7 +400,7 @@ def _per_leg(rows, counter: str, n_legs: int):
     return out
 
 
-def traffic_from(fetch_dir: str, write_dir: str, legs_json: str) -> int:
+def traffic_from(fetch_dir: str, write_dir: str, legs_json: str, out_path: str = TRAFFIC_PATH) -> int:
     """Builds profiles/traffic.json (HBM bytes per launch of every bench leg)
     from FETCH_SIZE and WRITE_SIZE counter runs of `bench.py --profile-legs`.
     Calibration: the memcpy_ceiling leg copies a known number of bytes each
@@ -441,7 +441,7 @@ def traffic_from(fetch_dir: str, write_dir: str, legs_json: str) -> int:
            "calibration": {"fetch_factor": round(f_fac, 4), "write_factor": round(w_fac, 4),
                            "bytes_each_way": int(each_way)},
            "kernels": kernels}
-    with open(TRAFFIC_PATH, "w") as f:
+    with open(out_path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")
     print(f"Kernel sources sha256 {out['kernel_sources_sha256'][:16]} (commit {commit}); calibration on the "
@@ -453,12 +453,12 @@ def traffic_from(fetch_dir: str, write_dir: str, legs_json: str) -> int:
     return 0
 
 
-def load_traffic():
+def load_traffic(path: str = TRAFFIC_PATH):
     """(kernels, stale): the committed per-leg HBM bytes, and whether they were
     measured on other kernel sources than these (then kernels is None)."""
-    if not os.path.exists(TRAFFIC_PATH):
+    if not os.path.exists(path):
         return None, True
-    with open(TRAFFIC_PATH) as f:
+    with open(path) as f:
         tr = json.load(f)
     if tr.get("kernel_sources_sha256") != kernel_sources_sha256():
         return None, True
@@ -481,6 +481,9 @@ def main():
 
     T.lib()  # fail loudly if the HIP library is missing
     assert torch.cuda.is_available(), "no GPU visible"
+    profiling = args.profile_legs is not None
+    if profiling:
+        ProfileLegs.legs = []
     if args.config == "batched":
         print(json.dumps({"metric": "batched gather jobs/s", "batched": bench_batched(args, torch, T)}), flush=True)
         return 0
@@ -531,20 +534,25 @@ def main():
     clock.collect()
     log(f"bench: timed {args.steps} steps in {wall:.3f}s")
 
-    step_bytes = sum(v[2] for v in clock.totals.values()) / args.steps
-    ms_per_step = wall * 1e3 / args.steps
-    value = step_bytes / (ms_per_step * 1e-3) / 1e9
-
     per = {}
+    if profiling:  # no timings: the legs run once each, marked
+        dname = None
+        value = ms_per_step = 0.0
+    else:
+        step_bytes = sum(v[2] for v in clock.totals.values()) / args.steps
+        ms_per_step = wall * 1e3 / args.steps
+        value = step_bytes / (ms_per_step * 1e-3) / 1e9
     for name, (launches, ms, nbytes) in clock.totals.items():
         avg_ms = ms / launches
         med_ms = statistics.median(clock.samples[name])
         per[name] = {"avg_ms": round(avg_ms, 4), "median_ms": round(med_ms, 4),
                      "gbs": round(nbytes / launches / (avg_ms * 1e-3) / 1e9, 1),
                      "median_gbs": round(nbytes / launches / (med_ms * 1e-3) / 1e9, 1)}
-    dominant = max(clock.totals.items(), key=lambda kv: kv[1][1])
-    dname, (dl, dms, dbytes) = dominant
-    d_gbs = dbytes / dl / (dms / dl * 1e-3) / 1e9
+    if not profiling:
+        dominant = max(clock.totals.items(), key=lambda kv: kv[1][1])
+        dname, (dl, dms, dbytes) = dominant
+        d_gbs = dbytes / dl / (dms / dl * 1e-3) / 1e9
+    r3, r2 = (1, 1) if profiling else (3, 2)  # launches per leg
 
     # Naive per-lane kernels and the plain-copy ceiling, same inputs, untimed
     # by the step clock.
@@ -554,14 +562,14 @@ def main():
         sums = torch.empty(n, dtype=torch.int32, device=dev)  # the load benchmark's per-struct sums
         for k in ks:
             s = 4 * k
-            for _ in range(3):
+            for _ in range(r3):
                 nclock.run(f"copy_direct_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="direct"))
                 nclock.run(f"gather_direct_k{k}", (2 * s + 4) * n,
                            lambda: T.gather(src, idx, dst, n, s, variant="direct"))
                 # The LDS-staged variant (trove/lds.h) beside the transposition.
                 nclock.run(f"copy_lds_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="lds"))
                 nclock.run(f"gather_lds_k{k}", (2 * s + 4) * n, lambda: T.gather(src, idx, dst, n, s, variant="lds"))
-            for _ in range(2):
+            for _ in range(r2):
                 nclock.run(f"scatter_tile_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="tile"))
                 nclock.run(f"scatter_shfl_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="shfl"))
                 nclock.run(f"scatter_direct_k{k}", (2 * s + 4) * n,
@@ -572,7 +580,7 @@ def main():
                 for v in ("shfl", "direct"):
                     nclock.run(f"store_{v}_k{k}", s * n, lambda v=v: T.store_counting(dst, n, s, variant=v))
                     nclock.run(f"load_{v}_k{k}", (s + 4) * n, lambda v=v: T.load_sum(src, sums, n, s, variant=v))
-        for _ in range(3):
+        for _ in range(r3):
             nclock.run("memcpy_ceiling", 2 * n * smax, lambda: T.memcpy_ceiling(src, dst, n * smax))
         # Block interface: 2^28 32-bit elements, s per lane, converged and
         # with one wavefront group in eight diverged.
@@ -581,9 +589,23 @@ def main():
         if not block_fits:
             log(f"bench: block interface leg skipped (buffers sized for --k {args.k})")
         for sb in (1, 3, 5, 7, 9, 11, 13, 15) if block_fits else ():
-            for _ in range(2):
+            for _ in range(r2):
                 nclock.run(f"block_s{sb}", 8 * e, lambda: T.block_copy(src, dst, e, sb))
                 nclock.run(f"block_div_s{sb}", 8 * e, lambda: T.block_copy(src, dst, e, sb, divergent=True, seed=7))
+        # Random row writes (rows of 32, 60, 64 and 128 bytes at the rows of a
+        # permutation, 2^26 rows up to 64 bytes, 2^25 of 128 bytes; ordinary
+        # and nontemporal stores): the rate the scatters' row writes are
+        # bounded by.
+        for rb in (32, 60, 64, 128):
+            n_w = n if rb <= 64 else n // 2
+            if n_w * rb > dst.numel() * 4 or n_w < 64:
+                continue
+            w_idx = idx if n_w == n else torch.from_numpy(T.gen_permutation(n_w, args.seed + 43)).to(dev)
+            for _ in range(r3):
+                for nt in (False, True):
+                    nclock.run(f"write_ceiling_{rb}_{'nt' if nt else 'cached'}", rb * n_w,
+                               lambda: T.random_rows_write_ceiling(dst, n_w, rb, w_idx, n_w, nontemporal=nt))
+            del w_idx
         # Random whole 128-byte lines of the 4 GiB source: the rate the
         # random gathers' line fills are bounded by (last: it leaves the
         # source's pages scattered through the caches and TLBs).
@@ -592,7 +614,7 @@ def main():
             n_reads = 1 << 26
             line_idx = torch.from_numpy(T.gen_indices(n_reads, n_lines, "random", args.seed + 41)).to(dev)
             line_out = torch.empty(n_reads, dtype=torch.int32, device=dev)
-            for _ in range(3):
+            for _ in range(r3):
                 nclock.run("random_lines_ceiling", 128 * n_reads,
                            lambda: T.random_lines_ceiling(src, n_lines, line_idx, line_out, n_reads))
             del line_idx, line_out
@@ -604,7 +626,7 @@ def main():
             naive[name] = {"avg_ms": round(avg_ms, 4), "gbs": round(nbytes / launches / (avg_ms * 1e-3) / 1e9, 1)}
 
     sweep = []
-    for k in ks:
+    for k in ks if not profiling else ():
         row = {"k": k, "bytes": 4 * k,
                "copy_gbs": per[f"copy_shfl_k{k}"]["gbs"],
                "gather_tile_gbs": per[f"gather_tile_k{k}"]["gbs"],
@@ -637,7 +659,7 @@ def main():
     # 192, 224, 384 and 768 bytes take the composite schedule), N = 2^24 up
     # to 256 bytes and 2^22 above.
     large = None
-    if naive:
+    if not args.no_naive:
         lclock = KernelClock(torch)
         for nl, sizes in ((1 << 24, (96, 128, 160, 192, 224, 256)), (1 << 22, (384, 512, 768, 1024))):
             # Up to 4 GiB of rows = the size of src / dst at the default sweep
@@ -647,7 +669,7 @@ def main():
                 continue
             idx_l = torch.from_numpy(T.gen_permutation(nl, args.seed + 31)).to(dev)
             for sb in sizes:
-                for _ in range(3):
+                for _ in range(r3):
                     lclock.run(f"copy_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="shfl"))
                     lclock.run(f"copy_naive_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="direct"))
                     lclock.run(f"gather_{sb}", (2 * sb + 4) * nl,
@@ -667,6 +689,11 @@ def main():
                              "N=2^24 up to 256 B and 2^22 above, gather through a seeded permutation of [0, N)"}
         for name, (launches, ms, nbytes) in lclock.totals.items():
             large[name + "_gbs"] = round(nbytes / launches / (ms / launches * 1e-3) / 1e9, 1)
+    if profiling:
+        with open(args.profile_legs, "w") as f:
+            json.dump({"legs": ProfileLegs.legs, "kernel_sources_sha256": kernel_sources_sha256()}, f)
+        log(f"bench: {len(ProfileLegs.legs)} marked legs written to {args.profile_legs}")
+        return 0
     block = None
     if naive and "block_s1" in naive:
         block = {"workload": "block interface copy, 2^28 x 32-bit elements, s per lane; divergent: 1 in 8 "
@@ -675,13 +702,32 @@ def main():
                  "divergent_gbs": {f"s{sb}": naive[f"block_div_s{sb}"]["gbs"] for sb in (1, 3, 5, 7, 9, 11, 13, 15)}}
 
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE passes (profiles/sweep_hbm_traffic.md).
-    traffic = None
-    traffic_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
-    if os.path.exists(traffic_path):
-        with open(traffic_path) as f:
-            tr = json.load(f).get("kernels", {})
-        traffic = tr.get(dname)
+    # FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json; null when those
+    # were measured on other kernel sources than these).
+    traffic_table, traffic_stale = load_traffic()
+    traffic = traffic_table.get(dname) if traffic_table else None
+    hbm_gbs = traffic / (dms / dl * 1e-3) / 1e9 if traffic else None
+    # The measured ceiling matching the dominant kernel: random line fills
+    # for a gather, the plain streaming copy for a copy.
+    ceiling_name = "random_lines_ceiling" if dname.startswith("gather") else "memcpy_ceiling"
+    ceiling_gbs = naive.get(ceiling_name, {}).get("gbs")
+    # Random-row write ceilings and the scatters of those row sizes against
+    # them (row bytes written per second by the tile scatter / by the ceiling
+    # kernel's better store policy).
+    write_ceiling = {}
+    for rb in (32, 60, 64, 128):
+        c, nt = naive.get(f"write_ceiling_{rb}_cached"), naive.get(f"write_ceiling_{rb}_nt")
+        if c and nt:
+            write_ceiling[str(rb)] = {"cached": c["gbs"], "nontemporal": nt["gbs"], "best": max(c["gbs"], nt["gbs"])}
+    scatter_vs_ceiling = {}
+    for k in ks:
+        rb = 4 * k
+        leg = naive.get(f"scatter_tile_k{k}")
+        if str(rb) in write_ceiling and leg:
+            row_gbs = rb * n / (leg["avg_ms"] * 1e-3) / 1e9
+            scatter_vs_ceiling[str(rb)] = {"kernel": f"scatter_tile_k{k}", "row_write_gbs": round(row_gbs, 1),
+                                           "ceiling_gbs": write_ceiling[str(rb)]["best"],
+                                           "ceiling_frac": round(row_gbs / write_ceiling[str(rb)]["best"], 3)}
 
     # Contiguous copy, small: 20-byte structs, N = 2^20 (21 MB: resident in
     # the 256 MB Infinity Cache, so this is a warm-cache number), checked
@@ -776,11 +822,17 @@ def main():
             "unit": "GB/s",
             "frac": round(d_gbs / HBM_PEAK_GBS, 3),
             "traffic": traffic,
-            "hbm_gbs": round(traffic / (dms / dl * 1e-3) / 1e9, 1) if traffic else None,
+            "traffic_stale": traffic_stale,
+            "hbm_gbs": round(hbm_gbs, 1) if hbm_gbs else None,
+            "ceiling": ceiling_name,
+            "ceiling_gbs": ceiling_gbs,
+            "ceiling_frac": round(hbm_gbs / ceiling_gbs, 3) if hbm_gbs and ceiling_gbs else None,
         },
         "cpu_baseline": cpu,
         "memcpy_ceiling_gbs": naive.get("memcpy_ceiling", {}).get("gbs"),
         "random_line_ceiling_gbs": naive.get("random_lines_ceiling", {}).get("gbs"),
+        "random_write_ceiling_gbs": write_ceiling or None,
+        "scatter_vs_write_ceiling": scatter_vs_ceiling or None,
         "block": block,
         "large_structs": large,
         "sweep": sweep,

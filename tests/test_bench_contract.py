@@ -41,6 +41,55 @@ def test_cpu_baseline_leg_on_a_small_sample():
     assert "N=2^12" in out["sample"]
 
 
+def _counter_csv(path, counter, dispatches):
+    # dispatches: (kernel name, grid size, KiB) in dispatch order
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write("Dispatch_Id,Kernel_Name,Grid_Size,Counter_Name,Counter_Value\n")
+        for d, (name, grid, kib) in enumerate(dispatches, start=1):
+            # Counters of several hardware instances appear as several rows.
+            f.write(f'{d},"{name}",{grid},{counter},{kib / 2}\n')
+            f.write(f'{d},"{name}",{grid},{counter},{kib / 2}\n')
+
+
+def test_traffic_from_attributes_dispatches_to_legs(tmp_path):
+    import bench
+
+    M = ("void trove_amd::k_memcpy16(uint4 const*, uint4*, long)", 256)
+    legs = [["copy_a", 2000], ["memcpy_ceiling", 8192], ["gather_b", 1000], ["copy_a", 2000]]
+    fetch = [("void trove_amd::k_fill_words(unsigned int*, long, unsigned long)", 4096, 99.0),  # set-up
+             (*M, 0.0), ("void trove_amd::k_copy<x>()", 1024, 1.0), ("void trove_amd::k_copy_tail<x>()", 64, 0.5),
+             (*M, 0.0), ("void trove_amd::k_memcpy16(uint4 const*, uint4*, long)", 8192, 2.0),
+             (*M, 0.0), ("at::native::elementwise_kernel<...>", 4096, 50.0), ("void trove_amd::k_gather<x>()", 512, 3.0),
+             (*M, 0.0), ("void trove_amd::k_copy<x>()", 1024, 2.5)]
+    write = [(n, g, 2 * k) for n, g, k in fetch]
+    _counter_csv(str(tmp_path / "f" / "host" / "1_counter_collection.csv"), "FETCH_SIZE", fetch)
+    _counter_csv(str(tmp_path / "w" / "host" / "1_counter_collection.csv"), "WRITE_SIZE", write)
+    with open(tmp_path / "legs.json", "w") as f:
+        json.dump({"legs": legs}, f)
+    out = tmp_path / "traffic.json"
+    assert bench.traffic_from(str(tmp_path / "f"), str(tmp_path / "w"), str(tmp_path / "legs.json"), str(out)) == 0
+    tr = json.load(open(out))
+    # memcpy_ceiling: 4096 bytes each way read 2 KiB fetched and 4 KiB written.
+    assert tr["calibration"]["fetch_factor"] == 2.0 and tr["calibration"]["write_factor"] == 1.0
+    k = tr["kernels"]
+    # copy_a: legs of 1.5 and 2.5 KiB fetched (the tail kernel counts), twice that written.
+    assert k["copy_a"] == round(2.0 * 1024 * 2 + 4.0 * 1024 * 1)
+    assert k["gather_b"] == round(3 * 1024 * 2 + 6 * 1024)  # torch's kernel is not the leg's
+    assert k["memcpy_ceiling"] == 8192
+    assert tr["kernel_sources_sha256"] == bench.kernel_sources_sha256()
+    kernels, stale = bench.load_traffic(str(out))
+    assert not stale and kernels == k
+    tr["kernel_sources_sha256"] = "0" * 64  # measured on other sources: not used
+    json.dump(tr, open(out, "w"))
+    assert bench.load_traffic(str(out)) == (None, True)
+    # A leg list that does not match the marked dispatches is refused.
+    with open(tmp_path / "legs.json", "w") as f:
+        json.dump({"legs": legs[:3]}, f)
+    with pytest.raises(SystemExit):
+        bench.traffic_from(str(tmp_path / "f"), str(tmp_path / "w"), str(tmp_path / "legs.json"), str(out))
+
+
 @pytest.mark.gpu
 def test_small_bench_run_keeps_the_contract(cuda):
     p = _bench("--k", "1,5", "--n-log2", "18", "--steps", "2", "--warmup", "0", "--no-cpu", "--jobs", "64",
