@@ -235,16 +235,24 @@ def bench_batched(args, torch, T):
     ba = T.Batch(W.jobs(choose, W.dst_a))
     bb = T.Batch(W.jobs(lambda s, p, L: tune.choose(table, s, p, L), W.dst_b))
     items, nbytes = ba.stats()
-    ba.run("persistent")
+    ba.run("launch")
     bb.run("looped")
     torch.cuda.synchronize()
     dst_a, dst_b = W.dst_a, W.dst_b
     same = all(torch.equal(dst_a[i:i + (1 << 26)], dst_b[i:i + (1 << 26)]) for i in range(0, dst_a.numel(), 1 << 26))
     reps = 5
-    t_pers = statistics.median(timed_runs(torch, lambda: ba.run("persistent"), reps))
+    t_pers = statistics.median(timed_runs(torch, lambda: ba.run("launch"), reps))
     ba.run("graph")
     t_graph = statistics.median(timed_runs(torch, lambda: ba.run("graph"), reps))
     t_loop = statistics.median(timed_runs(torch, lambda: bb.run("looped"), reps))
+    # The resident-grid (persistent) kernel, into dst_b, checked against the
+    # one-wavefront-per-item launch.
+    bb.run("resident")
+    torch.cuda.synchronize()
+    same = same and all(torch.equal(dst_a[i:i + (1 << 26)], dst_b[i:i + (1 << 26)])
+                        for i in range(0, dst_a.numel(), 1 << 26))
+    t_res = statistics.median(timed_runs(torch, lambda: bb.run("resident"), reps))
+    bound = batched_bound(torch, T, W, args.seed)
     # Spot-check a few jobs against the oracle on the host.
     import oracle
 
@@ -265,17 +273,56 @@ def bench_batched(args, torch, T):
         "workload": (f"{nj} gather jobs, struct 4..64 B uniform, length log-uniform 2^10..2^22, pattern uniform "
                      "{identity, random permutation, sorted runs mean 32}"),
         "jobs": nj, "structs": structs, "work_items": items, "algorithmic_bytes": nbytes,
-        "persistent_ms": round(t_pers, 3), "graph_ms": round(t_graph, 3), "looped_ms": round(t_loop, 3),
-        "persistent_jobs_per_s": round(nj / (t_pers * 1e-3), 1),
+        "modes": {"one_launch": "one launch, one wavefront per work item (TROVE_BATCH_LAUNCH)",
+                  "graph": "that launch replayed from a hipGraph (TROVE_BATCH_GRAPH)",
+                  "resident": "one launch of only the resident wavefronts, each striding over the work items "
+                              "(TROVE_BATCH_RESIDENT, a persistent kernel)",
+                  "looped": "one trove_gather launch per job (TROVE_BATCH_LOOPED)"},
+        "one_launch_ms": round(t_pers, 3), "graph_ms": round(t_graph, 3), "resident_ms": round(t_res, 3),
+        "looped_ms": round(t_loop, 3), "bound_ms": bound["ms"], "bound": bound["what"],
+        "one_launch_jobs_per_s": round(nj / (t_pers * 1e-3), 1),
         "graph_jobs_per_s": round(nj / (t_graph * 1e-3), 1),
+        "resident_jobs_per_s": round(nj / (t_res * 1e-3), 1),
         "looped_jobs_per_s": round(nj / (t_loop * 1e-3), 1),
-        "persistent_gbs": round(nbytes / (t_pers * 1e-3) / 1e9, 1),
+        "one_launch_gbs": round(nbytes / (t_pers * 1e-3) / 1e9, 1),
         "graph_gbs": round(nbytes / (t_graph * 1e-3) / 1e9, 1),
-        "speedup_persistent_vs_looped": round(t_loop / t_pers, 3),
+        "resident_gbs": round(nbytes / (t_res * 1e-3) / 1e9, 1),
+        "speedup_one_launch_vs_looped": round(t_loop / t_pers, 3),
         "speedup_graph_vs_looped": round(t_loop / t_graph, 3),
+        "speedup_resident_vs_looped": round(t_loop / t_res, 3),
+        "speedup_bound_vs_looped": round(t_loop / bound["ms"], 3),
         "variants_chosen": choice,
         "bit_exact": bool(same), "oracle_checked_jobs": checked,
     }
+
+
+def batched_bound(torch, T, W, seed):
+    """The batch priced job by job at standalone rates: every (struct size,
+    index pattern) is timed as one trove_gather of 2^22 structs (the largest
+    job length) through each gather path, and each job costs its algorithmic
+    bytes at the best of those rates. Their sum is what one launch would
+    take if each job ran at its pattern's large-job rate."""
+    L = 1 << 22
+    dev = torch.device("cuda:0")
+    src = torch.empty(L * 16, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+    T.fill_words_device(src, src.numel(), seed + 51)
+    rate = {}
+    for pattern in ("identity", "random", "runs"):
+        idx = torch.from_numpy(T.gen_indices(L, L, pattern, seed + 52)).to(dev)
+        for s in range(4, 65, 4):
+            best = None
+            for v in ("tile", "lds", "shfl", "direct"):
+                T.gather(src, idx, dst, L, s, variant=v)  # first launch loads the code object
+                ms = statistics.median(timed_runs(torch, lambda: T.gather(src, idx, dst, L, s, variant=v), 3))
+                best = ms if best is None else min(best, ms)
+            rate[(s, pattern)] = (2 * s + 4) * L / (best * 1e-3)
+        del idx
+    ms = sum((2 * int(s) + 4) * int(n) / rate[(int(s), p)] for s, n, p in zip(W.sizes, W.lengths, W.patterns)) * 1e3
+    del src, dst
+    return {"ms": round(ms, 3),
+            "what": "sum over jobs of algorithmic bytes / best standalone rate of the job's (size, pattern) "
+                    "at 2^22 structs over tile, LDS, coalesced_ptr and naive paths"}
 
 
 def bench_served(args, torch, T):

@@ -156,7 +156,7 @@ int trove_random_lines_ceiling(const void* buf, int64_t n_lines, const uint32_t*
 int trove_random_rows_write_ceiling(void* buf, int64_t n_rows, int row_bytes, const int32_t* row_idx,
                                     int64_t n_writes, int nontemporal, void* stream);
 
-/* ---- Batched jobs: many independent gathers / copies in one persistent launch ----
+/* ---- Batched jobs: many independent gathers / copies in one launch ----
  * No reference counterpart (the reference launches one kernel per
  * benchmark case, tests/benchmark.cu:143-145); a job with indices is the
  * gather of trove_gather, a job without (idx == NULL) the copy of
@@ -175,9 +175,12 @@ typedef struct trove_job {
 
 typedef struct trove_batch trove_batch; /* opaque plan */
 
-#define TROVE_BATCH_PERSISTENT 0 /* one launch walking every job's work items */
-#define TROVE_BATCH_GRAPH 1      /* that launch captured once as a hipGraph, then replayed */
-#define TROVE_BATCH_LOOPED 2     /* one trove_gather launch per job (the comparison) */
+#define TROVE_BATCH_LAUNCH 0   /* one launch, one wavefront per work item of every job */
+#define TROVE_BATCH_PERSISTENT TROVE_BATCH_LAUNCH /* the same mode's earlier name */
+#define TROVE_BATCH_GRAPH 1    /* the TROVE_BATCH_LAUNCH launch captured once as a hipGraph, then replayed */
+#define TROVE_BATCH_LOOPED 2   /* one trove_gather / trove_copy launch per job (the comparison) */
+#define TROVE_BATCH_RESIDENT 3 /* one launch of only the wavefronts resident at once, each striding over the
+                                  work items (a persistent kernel) */
 
 /* Validates the jobs and uploads the job table (allocates device memory). */
 int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out);

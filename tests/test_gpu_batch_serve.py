@@ -1,6 +1,7 @@
-"""GPU parity of the batched persistent launch and of the gather server.
+"""GPU parity of the batched launch and of the gather server.
 
-Batch: every job's output in the persistent, graph-replayed and looped modes
+Batch: every job's output in the one-launch, resident-grid, graph-replayed
+and looped modes
 equals the oracle's gather (or, for copy jobs without indices, copy) of that
 job (bit-exact). Graph replays are checked
 one by one with eager work and synchronisation between them.
@@ -52,7 +53,7 @@ def _check(jobs, what):
         assert np.array_equal(J["d_dst"].cpu().numpy(), J["want"]), (what, j, J["s"], J["n"], J["variant"])
 
 
-@pytest.mark.parametrize("mode", ["persistent", "looped"])
+@pytest.mark.parametrize("mode", ["launch", "resident", "looped"])
 def test_batch_modes(trove, cuda, mode):
     jobs = _jobs(96, 7)
     b = _device_jobs(jobs)
@@ -79,7 +80,7 @@ def test_batch_tiny_jobs_every_variant(trove, cuda):
                 jobs.append(dict(s=s, n=n, src=src, idx=None, want=oracle.copy(src, n, s), variant=variant,
                                  chunk=4))
     b = _device_jobs(jobs)
-    for mode in ("persistent", "looped"):
+    for mode in ("launch", "resident", "looped"):
         for J in jobs:
             J["d_dst"].zero_()
         torch.cuda.synchronize()
@@ -293,7 +294,7 @@ def test_serve_submit_validates_arrays(trove, cuda):
 def test_batched_config_full_shape(trove, cuda):
     # 1,024 jobs, sizes 4..64 B, lengths log-uniform 2^10..2^22, three index
     # patterns, variants and item sizes chosen by the autotuner: every job of
-    # the persistent launch equals torch indexing on the device and the
+    # the one launch equals torch indexing on the device and the
     # looped launches; 8 jobs equal the oracle; graph replays repeat it.
     from trove_amd import tune
     from trove_amd.workloads import BatchedWorkload
@@ -309,7 +310,7 @@ def test_batched_config_full_shape(trove, cuda):
 
     ba = T.Batch(W.jobs(choose, W.dst_a))
     bb = T.Batch(W.jobs(lambda s, p, L: tune.choose(table, s, p, L), W.dst_b))
-    ba.run("persistent")
+    ba.run("launch")
     bb.run("looped")
     torch.cuda.synchronize()
     assert len(chosen) >= 2 and W.lengths.min() >= 1 << 10 and W.lengths.max() <= 1 << 22
@@ -319,13 +320,18 @@ def test_batched_config_full_shape(trove, cuda):
         k, L = int(W.sizes[j]) // 4, int(W.lengths[j])
         i0 = W.idx_off[j]
         want = W.src[w0:w0 + nw].view(L, k)[idx_long[i0:i0 + L]]
-        assert torch.equal(W.dst_a[w0:w0 + nw].view(L, k), want), ("persistent", j)
+        assert torch.equal(W.dst_a[w0:w0 + nw].view(L, k), want), ("launch", j)
         assert torch.equal(W.dst_b[w0:w0 + nw], W.dst_a[w0:w0 + nw]), ("looped", j)
     for j in list(range(0, W.n_jobs, W.n_jobs // 8))[:8]:
         w0, nw = W.job_words(j)
         want = oracle.gather(W.src[w0:w0 + nw].cpu().numpy().view(np.uint8), W.job_indices(j), int(W.sizes[j]))
         assert np.array_equal(W.dst_a[w0:w0 + nw].cpu().numpy().view(np.uint8), want), ("oracle", j)
     ref = W.dst_a.clone()
+    W.dst_b.zero_()
+    bb.run("resident")
+    torch.cuda.synchronize()
+    assert all(torch.equal(ref[i:i + (1 << 26)], W.dst_b[i:i + (1 << 26)])
+               for i in range(0, ref.numel(), 1 << 26)), "resident"
     for replay in range(2):
         W.dst_a.zero_()
         torch.cuda.synchronize()

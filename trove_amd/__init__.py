@@ -313,14 +313,18 @@ class Job(ctypes.Structure):
                 ("chunk_groups", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
-BATCH_PERSISTENT = 0
+BATCH_LAUNCH = 0
+BATCH_PERSISTENT = BATCH_LAUNCH  # the same mode's earlier name
 BATCH_GRAPH = 1
 BATCH_LOOPED = 2
-BATCH_MODES = {"persistent": BATCH_PERSISTENT, "graph": BATCH_GRAPH, "looped": BATCH_LOOPED}
+BATCH_RESIDENT = 3
+BATCH_MODES = {"launch": BATCH_LAUNCH, "persistent": BATCH_LAUNCH, "graph": BATCH_GRAPH, "looped": BATCH_LOOPED,
+               "resident": BATCH_RESIDENT}
 
 
 class Batch:
-    """A validated set of gather and copy jobs runnable as one persistent launch."""
+    """A validated set of gather and copy jobs runnable as one launch (one
+    wavefront per work item, or resident wavefronts striding over them)."""
 
     def __init__(self, jobs):
         """jobs: iterable of (src, idx, dst, n, struct_bytes, variant, chunk_groups);
@@ -333,7 +337,7 @@ class Batch:
         _check(lib().trove_batch_create(ctypes.addressof(arr), len(jobs), ctypes.byref(h)), "trove_batch_create")
         self._h = h
 
-    def run(self, mode="persistent", stream=None) -> None:
+    def run(self, mode="launch", stream=None) -> None:
         m = BATCH_MODES[mode] if isinstance(mode, str) else int(mode)
         _check(lib().trove_batch_run(self._h, m, _stream(stream)), "trove_batch_run")
 

@@ -1,4 +1,4 @@
-// Job table of the persistent batch-gather kernel (batch.hip), shared with
+// Job table of the batch-gather kernels (batch.hip), shared with
 // the serving engine, which launches one table per step.
 #pragma once
 #include "common.h"
@@ -21,7 +21,7 @@ struct job_desc {
 };
 static_assert(sizeof(job_desc) == 56, "job_desc layout");
 
-// Launches the persistent gather over a device job table (sorted by
+// Launches the batch gather over a device job table (sorted by
 // first_item, total_items work items in all). lds: some job is LDS-staged
 // (the launch then carries a 4 KiB LDS stage per wavefront). flags
 // (nullable): flags[j] is set to 1 when job j meets an index above its

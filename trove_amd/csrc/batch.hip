@@ -1,6 +1,6 @@
 // Batched jobs: many independent gathers dst_j[i] = src_j[idx_j[i]] and
 // copies dst_j[i] = src_j[i] (idx_j == nullptr) of different struct sizes and
-// lengths in ONE persistent launch.
+// lengths in ONE launch.
 //
 // Work is cut into items of `chunk_groups` 64-struct groups of one job; item
 // t belongs to the job j with first_item[j] <= t < first_item[j + 1]. Every
@@ -24,7 +24,7 @@ namespace trove_amd {
 
 // Copy job items: whole 64-struct groups through load/store_warp_contiguous
 // (SHFL) or per lane (DIRECT); the partial last group per lane, so that the
-// persistent kernel carries no tile cascade.
+// batch kernel carries no tile cascade.
 template <typename T>
 __device__ __forceinline__ void copy_range(const T* __restrict__ src, T* __restrict__ dst, int64_t n, int64_t g0,
                                            int64_t g1, bool shfl) {
@@ -164,6 +164,30 @@ __global__ __launch_bounds__(kBlock) void k_batch_gather_pack(const job_pack p) 
     if (it < p.total_items) run_item(p.jobs, p.n_jobs, it, batch_stage(), p.flags);
 }
 
+// Resident-grid variant: a grid of only as many wavefronts as fit on the
+// device at once, each striding over the work items (t, t + W, t + 2W, ...,
+// W = wavefronts in the grid). The loop lets LICM keep every size's and
+// variant's per-lane tables live at once (162 VGPRs, 3 wavefronts per SIMD,
+// against 94 and 5 for one item per wavefront).
+__global__ __launch_bounds__(kBlock) void k_batch_resident(const job_desc* __restrict__ jobs, int n_jobs,
+                                                           int64_t total_items, int32_t* flags) {
+    for (int64_t it = wave_index(); it < total_items; it += wave_count()) run_item(jobs, n_jobs, it, batch_stage(), flags);
+}
+
+int launch_job_table_resident(const job_desc* d_table, int n_jobs, int64_t total_items, bool lds, int32_t* flags,
+                              hipStream_t s) {
+    const size_t stage = lds ? (size_t)kBatchStageBytes * kWavesPerBlock : 0;
+    int per_cu = 0;
+    int rc = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_resident, kBlock, stage));
+    if (rc != TROVE_OK) return rc;
+    const int64_t resident = (int64_t)(per_cu > 0 ? per_cu : 1) * device_cu_count();
+    const int64_t needed = (total_items + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t blocks = needed < resident ? needed : resident;
+    hipLaunchKernelGGL(k_batch_resident, dim3((unsigned)blocks), dim3(kBlock), stage, s, d_table, n_jobs,
+                       total_items, flags);
+    return launch_status();
+}
+
 int launch_job_table(const job_desc* d_table, int n_jobs, int64_t total_items, bool lds, int32_t* flags,
                      hipStream_t s) {
     // One wavefront per item: never capped.
@@ -202,7 +226,7 @@ static int batch_launch(trove_batch* b, hipStream_t s) {
     return launch_job_table(b->d_table, (int)b->table.size(), b->total_items, b->lds, nullptr, s);
 }
 
-// Captures the persistent launch into b->graph / b->graph_exec once. On any
+// Captures the one-launch mode into b->graph / b->graph_exec once. On any
 // failure the capture stream and graph are destroyed again, so that a later
 // call starts clean.
 static int capture_graph(trove_batch* b) {
@@ -303,7 +327,9 @@ int trove_batch_run(trove_batch* b, int mode, void* stream) {
     if (!b) return TROVE_ERR_INVALID;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (b->total_items == 0) return TROVE_OK;
-    if (mode == TROVE_BATCH_PERSISTENT) return batch_launch(b, s);
+    if (mode == TROVE_BATCH_LAUNCH) return batch_launch(b, s);
+    if (mode == TROVE_BATCH_RESIDENT)
+        return launch_job_table_resident(b->d_table, (int)b->table.size(), b->total_items, b->lds, nullptr, s);
     if (mode == TROVE_BATCH_GRAPH) {
         if (!b->graph_exec) {
             const int rc = capture_graph(b);

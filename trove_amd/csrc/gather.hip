@@ -21,7 +21,7 @@ namespace trove_amd {
 // 2^24 above), consistent over two runs: four groups at 60 and 128 bytes
 // (+3% and +4.5% over two), four groups in 64-thread work-groups at 64
 // bytes (+2.8%), 64-thread work-groups at 96 bytes (+1.5%); elsewhere
-// gather_groups_in_flight in 256-thread work-groups. (The persistent batch
+// gather_groups_in_flight in 256-thread work-groups. (The batch
 // kernel keeps gather_groups_in_flight: its registers are the maximum over
 // every size's branch.)
 template <typename T>

@@ -1,5 +1,5 @@
 // Per-wavefront gather bodies over a range of 64-struct groups, shared by the
-// standalone gather kernels and the persistent batch kernel.
+// standalone gather kernels and the batch kernels.
 //
 // A wavefront calls one of these with the same [g0, g1) on every lane;
 // group g covers structs [64 g, 64 g + 64) of a job of n structs, and the last

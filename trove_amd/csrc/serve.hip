@@ -11,7 +11,7 @@
 //    until a request slot and a range that fits are free;
 //  * continuous batching: each step gathers the next slice (at most
 //    `slice_indices`) of every active request in ONE launch of the
-//    persistent batch-gather kernel, so requests admitted while others are
+//    batch-gather kernel, so requests admitted while others are
 //    still on the GPU join the next step and their work overlaps on the
 //    device;
 //  * a step of up to 64 requests passes its job table in the kernel

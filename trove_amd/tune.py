@@ -3,7 +3,7 @@
 Picks, per (struct size, index pattern, length bucket), the kernel variant
 (tile path, coalesced_ptr path with the shuffle transposition, its LDS-staged
 counterpart, naive per-lane loads) and the work-item size
-(64-struct groups per wavefront item) of the persistent batch kernel, by
+(64-struct groups per wavefront item) of the one-launch batch kernel, by
 timing single-job batches of that shape on the device with HIP events. The
 table is plain JSON so a tuned device can be reused without re-timing.
 """
@@ -53,12 +53,12 @@ def autotune(sizes=SIZES, patterns=PATTERNS, lengths=LENGTHS, reps: int = 3, see
                 for variant in _variants(s):
                     for chunk in (CHUNKS if variant != "shfl" else (1,)):  # shfl items are one group
                         b = T.Batch([(src, idx, dst, L, s, variant, chunk)])
-                        b.run("persistent")  # warm-up
+                        b.run("launch")  # warm-up
                         times = []
                         for _ in range(reps):
                             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                             e0.record()
-                            b.run("persistent")
+                            b.run("launch")
                             e1.record()
                             e1.synchronize()
                             times.append(e0.elapsed_time(e1))
