@@ -186,6 +186,20 @@ typedef struct trove_batch trove_batch; /* opaque plan */
 int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out);
 /* Enqueues the whole batch on `stream` in the given mode. */
 int trove_batch_run(trove_batch* b, int mode, void* stream);
+/* Runs the jobs as trove_batch_create, trove_batch_run(TROVE_BATCH_LAUNCH)
+ * and trove_batch_destroy would, but allocates nothing: the job table goes
+ * into a static device array of the library (one per device, held until the
+ * process ends; TROVE_BATCH_STATIC_MAX_JOBS descriptors of 56 bytes, 224
+ * KiB), filled by small kernel launches that carry the descriptors in their
+ * arguments. No reference counterpart: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Jobs are validated as in
+ * trove_batch_create; more than TROVE_BATCH_STATIC_MAX_JOBS jobs return
+ * TROVE_ERR_FULL. Every call on one device shares that device's table:
+ * calls in order on one stream are safe, calls that overlap on one device
+ * (other streams, other host threads) race. */
+#define TROVE_BATCH_STATIC_MAX_JOBS 4096
+int trove_batch_run_static(const trove_job* jobs, int32_t n_jobs, void* stream);
 /* Work items and algorithmic bytes ((2 * struct_bytes + 4) * n per gather,
  * 2 * struct_bytes * n per copy, summed). */
 int trove_batch_stats(const trove_batch* b, int64_t* work_items, int64_t* algorithmic_bytes);

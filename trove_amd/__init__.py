@@ -118,6 +118,7 @@ _SIGNATURES = {
     "trove_batch_run": ([_vp, _int, _vp], _int),
     "trove_batch_stats": ([_vp, _vp, _vp], _int),
     "trove_batch_destroy": ([_vp], _int),
+    "trove_batch_run_static": ([_vp, ctypes.c_int32, _vp], _int),
     "trove_serve_create": ([_vp, _i64, ctypes.c_int32, _i64, ctypes.c_int32, _i64, ctypes.c_int32, _vp, _vp],
                            _int),
     "trove_serve_submit": ([_vp, _vp, _i64, _int, _vp], _int),
@@ -329,9 +330,7 @@ class Batch:
     def __init__(self, jobs):
         """jobs: iterable of (src, idx, dst, n, struct_bytes, variant, chunk_groups);
         idx None makes the job a contiguous copy."""
-        arr = (Job * len(jobs))()
-        for j, (src, idx, dst, n, b, variant, chunk) in enumerate(jobs):
-            arr[j] = Job(_addr(src), _addr(idx), _addr(dst), n, b, _variant(variant), chunk, 0)
+        arr = _job_array(jobs)
         self._jobs = arr
         h = ctypes.c_void_p()
         _check(lib().trove_batch_create(ctypes.addressof(arr), len(jobs), ctypes.byref(h)), "trove_batch_create")
@@ -356,6 +355,23 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+BATCH_STATIC_MAX_JOBS = 4096
+
+
+def _job_array(jobs):
+    arr = (Job * len(jobs))()
+    for j, (src, idx, dst, n, b, variant, chunk) in enumerate(jobs):
+        arr[j] = Job(_addr(src), _addr(idx), _addr(dst), n, b, _variant(variant), chunk, 0)
+    return arr
+
+
+def batch_run_static(jobs, stream=None) -> None:
+    """Runs jobs (tuples as for Batch) in one launch with the job table in the
+    library's static device array: no device allocation (trove_batch_run_static)."""
+    arr = _job_array(jobs)
+    _check(lib().trove_batch_run_static(ctypes.addressof(arr), len(jobs), _stream(stream)), "trove_batch_run_static")
 
 
 class Server:

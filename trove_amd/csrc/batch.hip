@@ -14,6 +14,7 @@
 #include "gather_body.h"
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -226,6 +227,80 @@ static int batch_launch(trove_batch* b, hipStream_t s) {
     return launch_job_table(b->d_table, (int)b->table.size(), b->total_items, b->lds, nullptr, s);
 }
 
+// Validates one job and builds its descriptor: the variant AUTO resolves to,
+// its work items (starting at first_item) and algorithmic bytes.
+static int describe_job(const trove_job& J, int64_t first_item, job_desc* d, int64_t* items, int64_t* bytes_out) {
+    const int bytes = J.struct_bytes;
+    if (bytes < 4 || bytes > 64 || bytes % 4 != 0) return TROVE_ERR_SIZE;
+    if (J.n < 0 || (J.n > 0 && (!J.src || !J.dst)) || J.reserved != 0) return TROVE_ERR_INVALID;
+    if (J.variant < TROVE_VARIANT_AUTO || J.variant > TROVE_VARIANT_LDS || J.chunk_groups < 0)
+        return TROVE_ERR_VARIANT;
+    const int w = word_bytes_of(bytes);
+    if (!aligned(J.src, w) || !aligned(J.dst, w) || !aligned(J.idx, 4)) return TROVE_ERR_ALIGN;
+    *d = job_desc{};
+    d->src = J.src;
+    d->idx = J.idx;
+    d->dst = J.dst;
+    d->n = J.n;
+    d->bytes = bytes;
+    d->last_row = 0xffffffffu;  // batch indices are the caller's to keep in range
+    const bool copy = J.idx == nullptr;
+    if (copy) {
+        // A copy job: SHFL (load/store_warp_contiguous), LDS
+        // (load/store_warp_contiguous_lds) or DIRECT; AUTO takes the
+        // library's per-size choice, TILE means SHFL.
+        const int v = J.variant == TROVE_VARIANT_AUTO ? auto_copy_variant(bytes) : J.variant;
+        d->variant = v == TROVE_VARIANT_TILE ? TROVE_VARIANT_SHFL : v;
+    } else {
+        d->variant = J.variant == TROVE_VARIANT_AUTO ? auto_gather_variant(bytes) : J.variant;
+    }
+    d->chunk_groups = J.chunk_groups > 0 ? J.chunk_groups : 16;
+    // coalesced_ptr gather items are one group: a loop around the tile
+    // cascade would keep every level's tables live.
+    if (!copy && d->variant == TROVE_VARIANT_SHFL) d->chunk_groups = 1;
+    d->first_item = first_item;
+    const int64_t groups = (J.n + 63) / 64;
+    *items = (groups + d->chunk_groups - 1) / d->chunk_groups;
+    *bytes_out = (2 * (int64_t)bytes + (copy ? 0 : 4)) * J.n;
+    return TROVE_OK;
+}
+
+// ---- trove_batch_run_static: the job table in a static device array ----
+//
+// `scratch` holds the job table of trove_batch_run_static: one array per
+// device, kept for the life of the process. Host code never passes its name
+// to a kernel or a copy (there it is only a host-side stand-in); it uses the
+// device address hipGetSymbolAddress gives, looked up once per device with
+// that device current.
+static __device__ __align__(256) job_desc scratch[TROVE_BATCH_STATIC_MAX_JOBS];
+
+constexpr int kMaxDevices = 64;
+static job_desc* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mutex;
+
+static int scratch_for_current_device(job_desc** out) {
+    int dev = 0;
+    int rc = hip_status(hipGetDevice(&dev));
+    if (rc != TROVE_OK) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return TROVE_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        rc = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (rc != TROVE_OK) return rc;
+        g_scratch_ptr[dev] = static_cast<job_desc*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return TROVE_OK;
+}
+
+// Writes up to kArgJobs descriptors, passed in the kernel arguments, to
+// table[first .. first + p.n_jobs).
+__global__ __launch_bounds__(64) void k_store_jobs(const job_pack p, int first, job_desc* table) {
+    const int j = (int)threadIdx.x;
+    if (j < p.n_jobs) table[first + j] = p.jobs[j];
+}
+
 // Captures the one-launch mode into b->graph / b->graph_exec once. On any
 // failure the capture stream and graph are destroyed again, so that a later
 // call starts clean.
@@ -261,51 +336,16 @@ int trove_batch_create(const trove_job* jobs, int32_t n_jobs, trove_batch** out)
     b->jobs.assign(jobs, jobs + n_jobs);
     int64_t item = 0;
     for (int j = 0; j < n_jobs; ++j) {
-        const trove_job& J = jobs[j];
-        const int bytes = J.struct_bytes;
-        if (bytes < 4 || bytes > 64 || bytes % 4 != 0) {
-            delete b;
-            return TROVE_ERR_SIZE;
-        }
-        if (J.n < 0 || (J.n > 0 && (!J.src || !J.dst)) || J.reserved != 0) {
-            delete b;
-            return TROVE_ERR_INVALID;
-        }
-        if (J.variant < TROVE_VARIANT_AUTO || J.variant > TROVE_VARIANT_LDS || J.chunk_groups < 0) {
-            delete b;
-            return TROVE_ERR_VARIANT;
-        }
-        const int w = word_bytes_of(bytes);
-        if (!aligned(J.src, w) || !aligned(J.dst, w) || !aligned(J.idx, 4)) {
-            delete b;
-            return TROVE_ERR_ALIGN;
-        }
         job_desc d{};
-        d.src = J.src;
-        d.idx = J.idx;
-        d.dst = J.dst;
-        d.n = J.n;
-        d.bytes = bytes;
-        d.last_row = 0xffffffffu;  // batch indices are the caller's to keep in range
-        const bool copy = J.idx == nullptr;
-        if (copy) {
-            // A copy job: SHFL (load/store_warp_contiguous), LDS
-            // (load/store_warp_contiguous_lds) or DIRECT; AUTO takes the
-            // library's per-size choice, TILE means SHFL.
-            const int v = J.variant == TROVE_VARIANT_AUTO ? auto_copy_variant(bytes) : J.variant;
-            d.variant = v == TROVE_VARIANT_TILE ? TROVE_VARIANT_SHFL : v;
-        } else {
-            d.variant = J.variant == TROVE_VARIANT_AUTO ? auto_gather_variant(bytes) : J.variant;
+        int64_t items = 0, bytes = 0;
+        const int rc = describe_job(jobs[j], item, &d, &items, &bytes);
+        if (rc != TROVE_OK) {
+            delete b;
+            return rc;
         }
-        d.chunk_groups = J.chunk_groups > 0 ? J.chunk_groups : 16;
-        // coalesced_ptr gather items are one group: a loop around the tile
-        // cascade would keep every level's tables live.
-        if (!copy && d.variant == TROVE_VARIANT_SHFL) d.chunk_groups = 1;
         b->lds = b->lds || d.variant == TROVE_VARIANT_LDS;
-        d.first_item = item;
-        const int64_t groups = (J.n + 63) / 64;
-        item += (groups + d.chunk_groups - 1) / d.chunk_groups;
-        b->algorithmic_bytes += (2 * (int64_t)bytes + (copy ? 0 : 4)) * J.n;
+        item += items;
+        b->algorithmic_bytes += bytes;
         b->table.push_back(d);
     }
     b->total_items = item;
@@ -347,6 +387,47 @@ int trove_batch_run(trove_batch* b, int mode, void* stream) {
         return TROVE_OK;
     }
     return TROVE_ERR_INVALID;
+}
+
+int trove_batch_run_static(const trove_job* jobs, int32_t n_jobs, void* stream) {
+    if (!jobs || n_jobs <= 0) return TROVE_ERR_INVALID;
+    if (n_jobs > TROVE_BATCH_STATIC_MAX_JOBS) return TROVE_ERR_FULL;
+    // Validate every job before anything is written to the device.
+    int64_t total_items = 0;
+    bool lds = false;
+    for (int j = 0; j < n_jobs; ++j) {
+        job_desc d;
+        int64_t items = 0, bytes = 0;
+        const int rc = describe_job(jobs[j], total_items, &d, &items, &bytes);
+        if (rc != TROVE_OK) return rc;
+        total_items += items;
+        lds = lds || d.variant == TROVE_VARIANT_LDS;
+    }
+    job_desc* table = nullptr;
+    int rc = scratch_for_current_device(&table);
+    if (rc != TROVE_OK) return rc;
+    if (total_items == 0) return TROVE_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // Descriptors travel to the table in the kernel arguments of one small
+    // launch per 64 jobs (stream-ordered before the batch launch).
+    job_pack p;
+    p.pad = 0;
+    p.total_items = 0;
+    p.flags = nullptr;
+    int64_t item = 0;
+    for (int first = 0; first < n_jobs; first += kArgJobs) {
+        const int cnt = n_jobs - first < kArgJobs ? n_jobs - first : kArgJobs;
+        for (int j = 0; j < cnt; ++j) {
+            int64_t items = 0, bytes = 0;
+            (void)describe_job(jobs[first + j], item, &p.jobs[j], &items, &bytes);
+            item += items;
+        }
+        p.n_jobs = cnt;
+        hipLaunchKernelGGL(k_store_jobs, dim3(1), dim3(64), 0, s, p, first, table);
+        rc = launch_status();
+        if (rc != TROVE_OK) return rc;
+    }
+    return launch_job_table(table, n_jobs, total_items, lds, nullptr, s);
 }
 
 int trove_batch_stats(const trove_batch* b, int64_t* work_items, int64_t* algorithmic_bytes) {
