@@ -607,6 +607,7 @@ def main():
     if not args.no_naive:
         nclock = KernelClock(torch)
         sums = torch.empty(n, dtype=torch.int32, device=dev)  # the load benchmark's per-struct sums
+        ws = torch.empty(n, dtype=torch.int32, device=dev)  # trove_scatter_permutation's workspace
         for k in ks:
             s = 4 * k
             for _ in range(r3):
@@ -622,6 +623,10 @@ def main():
                 nclock.run(f"scatter_direct_k{k}", (2 * s + 4) * n,
                            lambda: T.scatter(src, idx, dst, n, s, variant="direct"))
                 nclock.run(f"scatter_lds_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="lds"))
+                # The permutation scatter (inversion into ws + gather where
+                # that is faster; trove_scatter_permutation).
+                nclock.run(f"scatter_perm_k{k}", (2 * s + 4) * n,
+                           lambda: T.scatter_permutation(src, idx, dst, n, s, workspace=ws))
                 # The reference benchmark's store-only and load-only kernels
                 # (tests/benchmark.cu:52-88; bytes S and S + 4 per struct).
                 for v in ("shfl", "direct"):
@@ -667,7 +672,7 @@ def main():
             del line_idx, line_out
         torch.cuda.synchronize()
         nclock.collect()
-        del sums
+        del sums, ws
         for name, (launches, ms, nbytes) in nclock.totals.items():
             avg_ms = ms / launches
             naive[name] = {"avg_ms": round(avg_ms, 4), "gbs": round(nbytes / launches / (avg_ms * 1e-3) / 1e9, 1)}
@@ -690,6 +695,12 @@ def main():
             row.update({"scatter_tile_gbs": naive[f"scatter_tile_k{k}"]["gbs"],
                         "scatter_coalesced_ptr_gbs": naive[f"scatter_shfl_k{k}"]["gbs"],
                         "scatter_naive_gbs": naive[f"scatter_direct_k{k}"]["gbs"]})
+            best_scatter = max(row["scatter_tile_gbs"], row["scatter_coalesced_ptr_gbs"], row["scatter_naive_gbs"],
+                               naive[f"scatter_lds_k{k}"]["gbs"])
+            row["scatter_permutation"] = {"gbs": naive[f"scatter_perm_k{k}"]["gbs"],
+                                          "path": "invert + gather" if T.scatter_permutation_inverts(4 * k)
+                                                  else "scatter",
+                                          "vs_best_scatter": round(naive[f"scatter_perm_k{k}"]["gbs"] / best_scatter, 3)}
             row["lds_gbs"] = {"copy": naive[f"copy_lds_k{k}"]["gbs"], "gather": naive[f"gather_lds_k{k}"]["gbs"],
                               "scatter": naive[f"scatter_lds_k{k}"]["gbs"]}
             row["store_gbs"] = {"shfl": naive[f"store_shfl_k{k}"]["gbs"], "naive": naive[f"store_direct_k{k}"]["gbs"]}

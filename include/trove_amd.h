@@ -102,6 +102,28 @@ int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int 
 int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
                   void* stream);
 
+/* inv[perm[i]] = i, i < n, for a permutation perm of [0, n) (not checked).
+ * No reference counterpart: with it, a scatter through one permutation can
+ * be run as gathers through the inverse (see trove_scatter_permutation). */
+int trove_invert_permutation(const int32_t* perm, int32_t* inv, int64_t n, void* stream);
+
+/* dst[perm[i]] = src[i], i < n, for a permutation perm of [0, n) (dst holds
+ * n structs; not checked): the result of trove_scatter, computed where that
+ * is faster as trove_invert_permutation into workspace (n int32, device
+ * memory, 4-byte aligned) followed by trove_gather(src, workspace, dst)
+ * with the library's per-size path. That is the case for struct sizes of 12
+ * to 60 bytes (multiples of 4): random row writes that cover part of a
+ * 64-byte memory granule are several times slower than random row reads.
+ * Other sizes, or workspace == NULL, run trove_scatter(AUTO). The
+ * reference's random scatter benchmark scatters through a permutation
+ * (tests/benchmark.cu:101-110, 244-258). */
+int trove_scatter_permutation(const void* src, const int32_t* perm, void* dst, int64_t n, int struct_bytes,
+                              int32_t* workspace, void* stream);
+/* 1 when trove_scatter_permutation with a workspace inverts and gathers at
+ * this struct size, 0 when it scatters, TROVE_ERR_SIZE for a size the
+ * library does not compile. */
+int trove_scatter_permutation_inverts(int struct_bytes);
+
 /* dst[i] = counting_array(i * k) (k = struct_bytes / 4 words): the store
  * benchmark of tests/benchmark.cu:52-72 (SHFL: store_warp_contiguous<tile>;
  * DIRECT: plain stores). tile is the reference program's TILE_SIZE

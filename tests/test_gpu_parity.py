@@ -152,6 +152,32 @@ def test_scatter_all_variants(trove, cuda, s):
             assert np.array_equal(host(d_dst), want), (s, name, variant)
 
 
+def test_invert_permutation(trove, cuda):
+    for n in (1, 63, 64, 65, 64 * 1000 + 7):
+        perm = np.random.default_rng(n).permutation(n).astype(np.int32)
+        inv = torch.full((n,), -1, dtype=torch.int32, device=cuda)
+        T.invert_permutation(dev(perm), inv, n)
+        assert np.array_equal(host(inv), np.argsort(perm).astype(np.int32)), n
+
+
+@pytest.mark.parametrize("s", SIZES)
+def test_scatter_permutation_all_sizes(trove, cuda, s):
+    # dst[perm[i]] = src[i]: the oracle's scatter, through the inversion and
+    # a gather (12..60 bytes) or the plain scatter (other sizes, or no
+    # workspace); partial last group included.
+    n = 64 * 37 + 11
+    rng = np.random.default_rng(700 + s)
+    perm = rng.permutation(n).astype(np.int32)
+    src = rand_bytes(n * s, 800 + s)
+    want = oracle.scatter(src, perm, np.zeros(n * s, dtype=np.uint8), s)
+    d_src, d_perm = dev(src), dev(perm)
+    ws = torch.empty(n, dtype=torch.int32, device=cuda)
+    for workspace in (ws, None):
+        d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+        T.scatter_permutation(d_src, d_perm, d_dst, n, s, workspace=workspace)
+        assert np.array_equal(host(d_dst), want), (s, workspace is None)
+
+
 @pytest.mark.parametrize("s", [4, 8, 12, 16, 20, 60, 64, 96, 256])
 def test_gather_scatter_fewer_than_a_group(trove, cuda, s):
     # n below, at and just past one 64-struct group: every variant's partial
@@ -441,6 +467,16 @@ def test_full_size_sweep_properties(trove, cuda, sweep_inputs, k):
             assert torch.equal(dst2d[r0:r0 + step], src2d[S["idx_long"][r0:r0 + step]]), (variant, r0)
         got_rows = dst2d[torch.from_numpy(rows).long().to(cuda)].cpu().numpy()
         assert np.array_equal(got_rows, want_rows), variant
+    if k in (3, 8, 15, 16):
+        # The permutation scatter at full size: gathering back through the
+        # permutation returns the source.
+        ws = torch.empty(n, dtype=torch.int32, device=cuda)
+        T.scatter_permutation(d_src, S["d_idx"], d_dst, n, s, workspace=ws)
+        torch.cuda.synchronize()
+        dst2d = d_dst.view(n, k)
+        for r0 in range(0, n, step):
+            assert torch.equal(dst2d[S["idx_long"][r0:r0 + step]], src2d[r0:r0 + step]), ("scatter_permutation", r0)
+        del ws
 
 
 @pytest.mark.parametrize("s", [96, 256, 1024])

@@ -121,6 +121,15 @@ def test_validation_before_launch():
     assert L.trove_block_write(fake, 10, 0, None) == T.ERR_INVALID
     assert L.trove_memcpy_ceiling(fake, fake, 24, None) == T.ERR_INVALID
     assert L.trove_random_lines_ceiling(fake, 0, fake, fake, 10, None) == T.ERR_INVALID
+    assert [T.scatter_permutation_inverts(b) for b in (4, 8, 12, 32, 60, 64, 96, 6)] == \
+        [False, False, True, True, True, False, False, False]
+    assert L.trove_scatter_permutation_inverts(68) == T.ERR_SIZE
+    assert L.trove_invert_permutation(fake, fake, -1, None) == T.ERR_INVALID
+    assert L.trove_invert_permutation(fake + 2, fake, 10, None) == T.ERR_ALIGN
+    assert L.trove_invert_permutation(0, 0, 0, None) == T.OK
+    assert L.trove_scatter_permutation(fake, fake, fake, 10, 68, fake, None) == T.ERR_SIZE
+    assert L.trove_scatter_permutation(fake, fake, fake, 10, 20, fake + 2, None) == T.ERR_ALIGN
+    assert L.trove_scatter_permutation(fake, 0, fake, 10, 20, fake, None) == T.ERR_INVALID
     assert L.trove_random_rows_write_ceiling(fake, 100, 48, fake, 10, 0, None) == T.ERR_SIZE
     assert L.trove_random_rows_write_ceiling(fake, 0, 64, fake, 10, 0, None) == T.ERR_INVALID
     assert L.trove_random_rows_write_ceiling(fake + 4, 100, 64, fake, 10, 0, None) == T.ERR_ALIGN

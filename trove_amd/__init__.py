@@ -101,6 +101,9 @@ _SIGNATURES = {
     "trove_copy": ([_vp, _vp, _i64, _int, _int, _int, _vp], _int),
     "trove_gather": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
     "trove_scatter": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
+    "trove_invert_permutation": ([_vp, _vp, _i64, _vp], _int),
+    "trove_scatter_permutation": ([_vp, _vp, _vp, _i64, _int, _vp, _vp], _int),
+    "trove_scatter_permutation_inverts": ([_int], _int),
     "trove_store_counting": ([_vp, _i64, _int, _int, _int, _vp], _int),
     "trove_load_sum": ([_vp, _vp, _i64, _int, _int, _int, _vp], _int),
     "trove_block_copy": ([_vp, _vp, _i64, _int, _int, _u64, _vp], _int),
@@ -209,6 +212,26 @@ def scatter(src, idx, dst, n: int, struct_bytes: int, variant="tile", stream=Non
     """dst[idx[i]] = src[i] for i < n (idx distinct)."""
     _check(lib().trove_scatter(_addr(src), _addr(idx), _addr(dst), n, struct_bytes, _variant(variant),
                                _stream(stream)), "trove_scatter")
+
+
+def invert_permutation(perm, inv, n: int, stream=None) -> None:
+    """inv[perm[i]] = i for a permutation perm of [0, n)."""
+    _check(lib().trove_invert_permutation(_addr(perm), _addr(inv), n, _stream(stream)), "trove_invert_permutation")
+
+
+def scatter_permutation(src, perm, dst, n: int, struct_bytes: int, workspace=None, stream=None) -> None:
+    """dst[perm[i]] = src[i] for a permutation perm of [0, n); with a
+    workspace of n int32 it runs as an inversion and a gather where that is
+    faster."""
+    _check(lib().trove_scatter_permutation(_addr(src), _addr(perm), _addr(dst), n, struct_bytes, _addr(workspace),
+                                           _stream(stream)), "trove_scatter_permutation")
+
+
+def scatter_permutation_inverts(struct_bytes: int) -> bool:
+    """Whether scatter_permutation (with a workspace) inverts and gathers at this size."""
+    r = lib().trove_scatter_permutation_inverts(struct_bytes)
+    _check(r if r < 0 else OK, "trove_scatter_permutation_inverts")
+    return bool(r)
 
 
 def store_counting(dst, n: int, struct_bytes: int, variant="shfl", tile: int = 64, stream=None) -> None:

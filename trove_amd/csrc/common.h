@@ -40,6 +40,15 @@ constexpr int auto_gather_variant(int b) {
 constexpr int auto_scatter_variant(int b) {
     return b == 4 ? TROVE_VARIANT_DIRECT : (b == 8 || b == 16) ? TROVE_VARIANT_LDS : TROVE_VARIANT_TILE;
 }
+// trove_scatter_permutation: struct sizes where inverting the permutation
+// (one random 4-byte write per row) and gathering through the inverse beat
+// scattering the rows. Random row writes that cover part of a 64-byte
+// memory granule run far below whole-granule ones on MI355X (32-byte rows
+// 0.73 TB/s, 60-byte rows 0.87 TB/s, 64-byte rows 2.86 TB/s of written
+// bytes), while random row reads cost whole 128-byte line fills at ~5.4
+// TB/s; the sweep at 2^26 rows predicts 1.14-1.53x from 12 to 60 bytes and a
+// loss at 4, 8 and 64 bytes.
+constexpr bool auto_scatter_by_inverse(int b) { return b % 4 == 0 && b >= 12 && b <= 60; }
 constexpr int kWavesPerBlock = kBlock / 64;
 
 // The struct type moved for a given byte size: k 32-bit words when the size
