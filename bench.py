@@ -389,6 +389,15 @@ def bench_served(args, torch, T):
             "rate_req_per_s": round(rate, 1), "gbs": round(total_bytes / wall / 1e9, 1),
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"]}
+    # The same arrivals at fixed absolute rates, comparable across builds
+    # whose saturation rates differ.
+    for rate in (5e5, 1e6, 1.5e6):
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=R))
+        lat, wall, st = run(arrivals)
+        out[f"poisson_{rate / 1e6:g}M_per_s"] = {
+            "rate_req_per_s": rate, "gbs": round(total_bytes / wall / 1e9, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"]}
     # Bit-exactness: a few requests alone vs the oracle.
     import oracle
 

@@ -78,7 +78,7 @@ constexpr int kBatchStageBytes = 64 * 64;
 
 template <int B>
 __device__ __forceinline__ void batch_item(const job_desc& J0, int64_t local, uint32_t* stage, int32_t* flag) {
-    const row_guard guard{J0.last_row, flag};
+    row_guard guard{J0.last_row, flag};
     typedef blob_t<B> T;
     job_desc J = J0;
     J.src = assume_global(J.src);

@@ -10,22 +10,30 @@
 namespace trove_amd {
 
 // Index guards: every gather body passes each index it reads through one
-// before using it as a row. no_guard trusts the caller (the C ABI's
-// gathers); row_guard maps an index outside [0, last] to row 0 and records
-// the violation in *flag (when set), so that a job with a bad index never
-// reads outside its source. The flag is written with a system-scope store so
-// that it may live in pinned host memory.
+// before using it as a row, and calls report() once at its end. no_guard
+// trusts the caller (the C ABI's gathers). row_guard maps an index outside
+// [0, last] to row 0 with a select (no branch between an index load and the
+// next one: a branch there would wait for each index before loading the
+// next, one group at a time) and remembers it per lane; report() then sets
+// *flag (when given), so that a job with a bad index never reads outside
+// its source. The flag is written with a system-scope store so that it may
+// live in pinned host memory. A body's guard is its own copy.
 struct no_guard {
-    __device__ __forceinline__ int32_t operator()(int32_t ix) const { return ix; }
+    __device__ __forceinline__ int32_t operator()(int32_t ix) { return ix; }
+    __device__ __forceinline__ void report() const {}
 };
 
 struct row_guard {
-    uint32_t last;  // largest valid row (0xffffffff: every index valid)
-    int32_t* flag;  // nullable
-    __device__ __forceinline__ int32_t operator()(int32_t ix) const {
-        if ((uint32_t)ix <= last) return ix;
-        if (flag) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        return 0;
+    uint32_t last;      // largest valid row (0xffffffff: every index valid)
+    int32_t* flag;      // nullable
+    bool bad = false;   // this lane met an index above last
+    __device__ __forceinline__ int32_t operator()(int32_t ix) {
+        const bool ok = (uint32_t)ix <= last;
+        bad = bad || !ok;
+        return ok ? ix : 0;
+    }
+    __device__ __forceinline__ void report() const {
+        if (bad && flag) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 };
 
@@ -80,6 +88,7 @@ __device__ __forceinline__ void gather_range_word(const T* __restrict__ src, con
         const int64_t i = (full << 6) + lane;
         if (i < n) d[i] = s[guard(idx[i])];
     }
+    guard.report();
 }
 
 template <typename T, int UNROLL, typename G = no_guard>
@@ -115,6 +124,7 @@ __device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, con
         const int64_t i = (full << 6) + lane;
         if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + guard(idx[i])), dst + i);
     }
+    guard.report();
 }
 
 // coalesced_ptr path (tests/benchmark.cu:90-99) for ONE group g: a partial
@@ -131,6 +141,7 @@ __device__ __forceinline__ void gather_group_shfl(const T* __restrict__ src, con
         const T v = s[guard(idx[i])];
         r[i] = v;
     }
+    guard.report();
 }
 
 // LDS-staged path (trove/lds.h) for ONE full group g, the LDS counterpart of
@@ -163,6 +174,7 @@ __device__ __forceinline__ void gather_group_lds(const T* __restrict__ src, cons
     d::lds_chunks<T, CW> t;
     d::lds_get_chunks(stage, lane, t);
     d::lds_global_store<CW, trove::streaming>(dst + (g << 6), lane, t);
+    guard.report();
 }
 
 // LDS-staged gather over groups [g0, g1); the partial last group per lane.
@@ -181,6 +193,7 @@ __device__ __forceinline__ void gather_range_lds(const T* __restrict__ src, cons
         const int64_t i = (full << 6) + trove::lane_id();
         if (i < n) trove::detail::divergent_store(trove::detail::divergent_load(src + guard(idx[i])), dst + i);
     }
+    guard.report();
 }
 
 // Naive per-lane gather (tests/benchmark.cu:112-119).
@@ -193,6 +206,7 @@ __device__ __forceinline__ void gather_range_direct(const T* __restrict__ src, c
         const int64_t i = (g << 6) + lane;
         if (i < n) dst[i] = src[guard(idx[i])];
     }
+    guard.report();
 }
 
 }  // namespace trove_amd

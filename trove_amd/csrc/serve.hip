@@ -468,13 +468,19 @@ static int replay_loop(trove_serve* s, int32_t n_req, const double* arrival_s, c
     std::vector<uint64_t> ids(n_req, 0);
     std::unordered_map<uint64_t, int32_t> slot_of;
     slot_of.reserve((size_t)n_req * 2);
+    s->reqs.reserve(s->reqs.size() + (size_t)n_req);
     int32_t next = 0, done = 0;
     // A rejected request (an index outside the table) reports latency -1.
     auto latency_of = [](const request& r) { return r.bad_index ? -1.0 : r.t_done - r.t_submit; };
     const double t0 = now_s();
     while (done < n_req) {
         const double t = now_s() - t0;
-        while (next < n_req && arrival_s[next] <= t) {
+        // At most max_slots submissions between steps: admission cannot take
+        // more at once, and a burst of due arrivals (a saturation run) then
+        // reaches the GPU after one slot's worth of host work instead of all
+        // of it. Latency still counts from each request's scheduled arrival.
+        int32_t burst = s->max_slots;
+        while (next < n_req && arrival_s[next] <= t && burst-- > 0) {
             const int rc = trove_serve_submit(s, idx[next], n[next], idx_on_device, &ids[next]);
             if (rc != TROVE_OK) return rc;
             // Latency counts from the scheduled arrival, not from submission.
