@@ -59,6 +59,9 @@ def parse_args():
     p.add_argument("--traffic-from", nargs=3, default=None, metavar=("FETCH_DIR", "WRITE_DIR", "LEGS_JSON"),
                    help="no GPU: build profiles/traffic.json from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                         "output directories of a --profile-legs run, print the table as markdown")
+    p.add_argument("--counters-from", nargs="+", default=None, metavar="DIR",
+                   help="no GPU: DIR... LEGS_JSON: print every counter of rocprofv3 --pmc runs of a "
+                        "--profile-legs run per leg (mean over the leg's launches) as a markdown table")
     return p.parse_args()
 
 
@@ -456,6 +459,31 @@ def _per_leg(rows, counter: str, n_legs: int):
     return out
 
 
+def counters_from(dirs, legs_json: str) -> int:
+    """Per-leg means of every counter found in the counter runs under dirs
+    (each a --pmc run of the same --profile-legs program), as markdown."""
+    with open(legs_json) as f:
+        legs = json.load(f)["legs"]
+    table = {}  # counter -> per-leg values
+    for d in dirs:
+        rows = _counter_rows(d)
+        for counter in sorted({r["Counter_Name"] for r in rows}):
+            table[counter] = _per_leg(rows, counter, len(legs))
+    counters = sorted(table)
+    per = {}
+    for i, (name, _) in enumerate(legs):
+        rec = per.setdefault(name, [0] + [0.0] * len(counters))
+        rec[0] += 1
+        for c, counter in enumerate(counters):
+            rec[1 + c] += table[counter][i]
+    print("| leg | " + " | ".join(counters) + " |")
+    print("|---" * (len(counters) + 1) + "|")
+    for name in sorted(per):
+        rec = per[name]
+        print(f"| `{name}` | " + " | ".join(f"{v / rec[0]:.4g}" for v in rec[1:]) + " |")
+    return 0
+
+
 def traffic_from(fetch_dir: str, write_dir: str, legs_json: str, out_path: str = TRAFFIC_PATH) -> int:
     """Builds profiles/traffic.json (HBM bytes per launch of every bench leg)
     from FETCH_SIZE and WRITE_SIZE counter runs of `bench.py --profile-legs`.
@@ -525,6 +553,10 @@ def main():
     args = parse_args()
     if args.traffic_from:
         return traffic_from(*args.traffic_from)
+    if args.counters_from:
+        if len(args.counters_from) < 2:
+            raise SystemExit("--counters-from needs at least one counter directory and the legs JSON")
+        return counters_from(args.counters_from[:-1], args.counters_from[-1])
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))

@@ -83,6 +83,16 @@ def test_traffic_from_attributes_dispatches_to_legs(tmp_path):
     tr["kernel_sources_sha256"] = "0" * 64  # measured on other sources: not used
     json.dump(tr, open(out, "w"))
     assert bench.load_traffic(str(out)) == (None, True)
+    # The generic per-leg counter table reads the same runs.
+    import contextlib
+    import io
+
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        assert bench.counters_from([str(tmp_path / "f"), str(tmp_path / "w")], str(tmp_path / "legs.json")) == 0
+    lines = buf.getvalue().splitlines()
+    assert lines[0] == "| leg | FETCH_SIZE | WRITE_SIZE |"
+    assert "| `copy_a` | 2 | 4 |" in lines and "| `gather_b` | 3 | 6 |" in lines
     # A leg list that does not match the marked dispatches is refused.
     with open(tmp_path / "legs.json", "w") as f:
         json.dump({"legs": legs[:3]}, f)
