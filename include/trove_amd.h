@@ -90,16 +90,20 @@ int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int vari
  * stores with the transposition elided (4-, 8- and 16-byte structs: per-lane
  * row words); LDS: the tile-indexed loads staged through LDS to one struct
  * per lane and back out with dwordx4 stores (the LDS-staged counterpart of
- * SHFL). AUTO: LDS for 16, 48, 64 and 192 bytes, else TILE. */
-int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
+ * SHFL). AUTO: LDS for 16, 48, 64 and 192 bytes, else TILE.
+ * tile: 0 or 64 = whole wavefronts; a power of two 2..32 runs SHFL (also for
+ * AUTO) with each converged tile of that many lanes on its own, as the
+ * reference's coalesced_ptr runs on a 32-lane warp; TILE and LDS need whole
+ * wavefronts (TROVE_ERR_VARIANT otherwise), DIRECT ignores it. */
+int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant, int tile,
                  void* stream);
 
 /* dst[idx[i]] = src[i], i < n, idx distinct. SHFL: tests/benchmark.cu:101-110;
  * DIRECT: tests/benchmark.cu:121-128; TILE: coalesced loads + tile-indexed
  * stores with the transposition elided; LDS: dwordx4 loads staged through
  * LDS to one struct per lane, then tile-indexed stores. AUTO: DIRECT for 4
- * bytes, LDS for 8 and 16, else TILE. */
-int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
+ * bytes, LDS for 8 and 16, else TILE. tile as for trove_gather. */
+int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant, int tile,
                   void* stream);
 
 /* inv[perm[i]] = i, i < n, for a permutation perm of [0, n) (not checked).

@@ -148,8 +148,10 @@ void random_access(const options& o, const device_buffer& perm, const char* name
     device_buffer r(src.bytes);
     const int32_t* idx = static_cast<const int32_t*>(perm.p);
     auto run = [&](void* out, int v) {
-        return gather ? trove_gather(src.p, idx, out, o.n, s, v, nullptr)
-                      : trove_scatter(src.p, idx, out, o.n, s, v, nullptr);
+        // The reference's gathers and scatters go through coalesced_ptr, which
+        // has no tile width: whole wavefronts here.
+        return gather ? trove_gather(src.p, idx, out, o.n, s, v, 0, nullptr)
+                      : trove_scatter(src.p, idx, out, o.n, s, v, 0, nullptr);
     };
     const float ms = time_ms([&] { trove_ok(run(r.p, variant), name); }, o.iterations);
     bool correct = true;

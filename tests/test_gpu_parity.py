@@ -152,6 +152,28 @@ def test_scatter_all_variants(trove, cuda, s):
             assert np.array_equal(host(d_dst), want), (s, name, variant)
 
 
+@pytest.mark.parametrize("s", [8, 12, 20, 32, 60, 64, 96, 256, 1024, 6])
+def test_gather_scatter_sub_wavefront_tiles(trove, cuda, s):
+    # trove_gather / trove_scatter with tile 32..2: coalesced_ptr on tiles of
+    # that many lanes (AUTO resolves to it), a partial last group included;
+    # DIRECT ignores the width.
+    n = 64 * 29 + 37
+    rng = np.random.default_rng(900 + s)
+    src = rand_bytes(n * s, 910 + s)
+    perm = rng.permutation(n).astype(np.int32)
+    want_g = oracle.gather(src, perm, s)
+    want_s = oracle.scatter(src, perm, np.zeros(n * s, dtype=np.uint8), s)
+    d_src, d_perm = dev(src), dev(perm)
+    for tile in (32, 16, 8, 4, 2):
+        for variant in ("shfl", "auto", "direct"):
+            d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
+            T.gather(d_src, d_perm, d_dst, n, s, variant=variant, tile=tile)
+            assert np.array_equal(host(d_dst), want_g), ("gather", s, tile, variant)
+            d_dst.zero_()
+            T.scatter(d_src, d_perm, d_dst, n, s, variant=variant, tile=tile)
+            assert np.array_equal(host(d_dst), want_s), ("scatter", s, tile, variant)
+
+
 def test_invert_permutation(trove, cuda):
     for n in (1, 63, 64, 65, 64 * 1000 + 7):
         perm = np.random.default_rng(n).permutation(n).astype(np.int32)

@@ -73,8 +73,8 @@ def test_lds_variant_limits():
     for b in (768, 1024, 2, 6):
         assert not T.lds_supported(b)
         assert L.trove_copy(fake, fake, 10, b, T.VARIANT_LDS, 64, None) == T.ERR_VARIANT
-        assert L.trove_gather(fake, fake, fake, 10, b, T.VARIANT_LDS, None) == T.ERR_VARIANT
-        assert L.trove_scatter(fake, fake, fake, 10, b, T.VARIANT_LDS, None) == T.ERR_VARIANT
+        assert L.trove_gather(fake, fake, fake, 10, b, T.VARIANT_LDS, 0, None) == T.ERR_VARIANT
+        assert L.trove_scatter(fake, fake, fake, 10, b, T.VARIANT_LDS, 0, None) == T.ERR_VARIANT
     assert L.trove_store_counting(fake, 10, 20, T.VARIANT_LDS, 0, None) == T.ERR_VARIANT
     assert L.trove_load_sum(fake, fake, 10, 20, T.VARIANT_LDS, 0, None) == T.ERR_VARIANT
     assert [b for b in T.STRUCT_SIZES if T.lds_supported(b)] == \
@@ -106,13 +106,20 @@ def test_validation_before_launch():
     assert L.trove_copy(fake + 4, fake, 10, 32, 1, 64, None) == T.ERR_ALIGN
     assert L.trove_copy(0, fake, 10, 20, 1, 64, None) == T.ERR_INVALID
     assert L.trove_copy(0, 0, 0, 20, 1, 64, None) == T.OK  # nothing to do
-    assert L.trove_gather(fake, fake + 2, fake, 10, 20, 3, None) == T.ERR_ALIGN
-    assert L.trove_gather(fake, 0, fake, 10, 20, 3, None) == T.ERR_INVALID
-    assert L.trove_scatter(fake, fake, fake, 10, 20, 9, None) == T.ERR_VARIANT
-    assert L.trove_gather(fake, fake, fake, 10, 20, T.VARIANT_LDS + 1, None) == T.ERR_VARIANT
-    assert L.trove_gather(0, 0, 0, 0, 20, 3, None) == T.OK
+    assert L.trove_gather(fake, fake + 2, fake, 10, 20, 3, 0, None) == T.ERR_ALIGN
+    assert L.trove_gather(fake, 0, fake, 10, 20, 3, 0, None) == T.ERR_INVALID
+    assert L.trove_scatter(fake, fake, fake, 10, 20, 9, 0, None) == T.ERR_VARIANT
+    assert L.trove_gather(fake, fake, fake, 10, 20, T.VARIANT_LDS + 1, 0, None) == T.ERR_VARIANT
+    assert L.trove_gather(0, 0, 0, 0, 20, 3, 0, None) == T.OK
     assert L.trove_store_counting(fake, 10, 6, 1, 0, None) == T.ERR_SIZE
     assert L.trove_load_sum(fake, fake + 1, 10, 20, 1, 0, None) == T.ERR_ALIGN
+    for bad_tile in (1, 3, 48, 128, -64):  # gather / scatter tile widths
+        assert L.trove_gather(fake, fake, fake, 10, 20, 1, bad_tile, None) == T.ERR_VARIANT
+        assert L.trove_scatter(fake, fake, fake, 10, 20, 1, bad_tile, None) == T.ERR_VARIANT
+    # Sub-wavefront tiles are coalesced_ptr's: not for the tile or LDS-staged paths.
+    for v in (T.VARIANT_TILE, T.VARIANT_LDS):
+        assert L.trove_gather(fake, fake, fake, 10, 20, v, 32, None) == T.ERR_VARIANT
+        assert L.trove_scatter(fake, fake, fake, 10, 20, v, 16, None) == T.ERR_VARIANT
     for bad_tile in (1, 3, 48, 128, -64):  # the store / load benchmarks' tile widths
         assert L.trove_store_counting(fake, 10, 20, 1, bad_tile, None) == T.ERR_VARIANT
         assert L.trove_load_sum(fake, fake, 10, 20, 1, bad_tile, None) == T.ERR_VARIANT

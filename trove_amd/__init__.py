@@ -99,8 +99,8 @@ _SIGNATURES = {
     "trove_struct_word_bytes": ([_int], _int),
     "trove_auto_variant": ([_int, _int], _int),
     "trove_copy": ([_vp, _vp, _i64, _int, _int, _int, _vp], _int),
-    "trove_gather": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
-    "trove_scatter": ([_vp, _vp, _vp, _i64, _int, _int, _vp], _int),
+    "trove_gather": ([_vp, _vp, _vp, _i64, _int, _int, _int, _vp], _int),
+    "trove_scatter": ([_vp, _vp, _vp, _i64, _int, _int, _int, _vp], _int),
     "trove_invert_permutation": ([_vp, _vp, _i64, _vp], _int),
     "trove_scatter_permutation": ([_vp, _vp, _vp, _i64, _int, _vp, _vp], _int),
     "trove_scatter_permutation_inverts": ([_int], _int),
@@ -202,15 +202,15 @@ def copy(src, dst, n: int, struct_bytes: int, variant="shfl", tile: int = 64, st
            "trove_copy")
 
 
-def gather(src, idx, dst, n: int, struct_bytes: int, variant="tile", stream=None) -> None:
-    """dst[i] = src[idx[i]] for i < n."""
-    _check(lib().trove_gather(_addr(src), _addr(idx), _addr(dst), n, struct_bytes, _variant(variant),
+def gather(src, idx, dst, n: int, struct_bytes: int, variant="tile", tile: int = 64, stream=None) -> None:
+    """dst[i] = src[idx[i]] for i < n (tile < 64: coalesced_ptr on tiles of that many lanes)."""
+    _check(lib().trove_gather(_addr(src), _addr(idx), _addr(dst), n, struct_bytes, _variant(variant), tile,
                               _stream(stream)), "trove_gather")
 
 
-def scatter(src, idx, dst, n: int, struct_bytes: int, variant="tile", stream=None) -> None:
-    """dst[idx[i]] = src[i] for i < n (idx distinct)."""
-    _check(lib().trove_scatter(_addr(src), _addr(idx), _addr(dst), n, struct_bytes, _variant(variant),
+def scatter(src, idx, dst, n: int, struct_bytes: int, variant="tile", tile: int = 64, stream=None) -> None:
+    """dst[idx[i]] = src[i] for i < n (idx distinct; tile as for gather)."""
+    _check(lib().trove_scatter(_addr(src), _addr(idx), _addr(dst), n, struct_bytes, _variant(variant), tile,
                                _stream(stream)), "trove_scatter")
 
 

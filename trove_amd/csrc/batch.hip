@@ -380,7 +380,7 @@ int trove_batch_run(trove_batch* b, int mode, void* stream) {
     if (mode == TROVE_BATCH_LOOPED) {
         for (const job_desc& d : b->table) {
             if (d.n == 0) continue;
-            const int rc = d.idx ? trove_gather(d.src, d.idx, d.dst, d.n, d.bytes, d.variant, stream)
+            const int rc = d.idx ? trove_gather(d.src, d.idx, d.dst, d.n, d.bytes, d.variant, 64, stream)
                                  : trove_copy(d.src, d.dst, d.n, d.bytes, d.variant, 64, stream);
             if (rc != TROVE_OK) return rc;
         }

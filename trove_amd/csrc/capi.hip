@@ -85,7 +85,7 @@ int64_t grid_for(int64_t waves, int waves_per_block) {
 }
 
 typedef int (*copy_fn)(const void*, void*, int64_t, int, int, hipStream_t);
-typedef int (*index_fn)(const void*, const int32_t*, void*, int64_t, int, hipStream_t);
+typedef int (*index_fn)(const void*, const int32_t*, void*, int64_t, int, int, hipStream_t);
 typedef int (*counting_fn)(void*, int64_t, int, int, hipStream_t);
 typedef int (*loadsum_fn)(const void*, uint32_t*, int64_t, int, int, hipStream_t);
 
@@ -188,22 +188,24 @@ static int index_common(const size_entry* e, const void* src, const int32_t* idx
     return TROVE_OK;
 }
 
-int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
+int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant, int tile,
                  void* stream) {
     const size_entry* e = find_size(struct_bytes);
     bool launch = false;
+    if (!valid_tile(tile)) return TROVE_ERR_VARIANT;
     const int rc = index_common(e, src, idx, dst, n, struct_bytes, variant, &launch);
     if (rc != TROVE_OK || !launch) return rc;
-    return e->gather(src, idx, dst, n, variant, static_cast<hipStream_t>(stream));
+    return e->gather(src, idx, dst, n, variant, tile, static_cast<hipStream_t>(stream));
 }
 
-int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant,
+int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant, int tile,
                   void* stream) {
     const size_entry* e = find_size(struct_bytes);
     bool launch = false;
+    if (!valid_tile(tile)) return TROVE_ERR_VARIANT;
     const int rc = index_common(e, src, idx, dst, n, struct_bytes, variant, &launch);
     if (rc != TROVE_OK || !launch) return rc;
-    return e->scatter(src, idx, dst, n, variant, static_cast<hipStream_t>(stream));
+    return e->scatter(src, idx, dst, n, variant, tile, static_cast<hipStream_t>(stream));
 }
 
 int trove_invert_permutation(const int32_t* perm, int32_t* inv, int64_t n, void* stream) {
@@ -227,11 +229,11 @@ int trove_scatter_permutation(const void* src, const int32_t* perm, void* dst, i
     if (rc != TROVE_OK || !launch) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (!workspace || !auto_scatter_by_inverse(struct_bytes))
-        return e->scatter(src, perm, dst, n, TROVE_VARIANT_AUTO, s);
+        return e->scatter(src, perm, dst, n, TROVE_VARIANT_AUTO, 0, s);
     if (!aligned(workspace, 4)) return TROVE_ERR_ALIGN;
     const int irc = invert_permutation_launch(perm, workspace, n, s);
     if (irc != TROVE_OK) return irc;
-    return e->gather(src, workspace, dst, n, TROVE_VARIANT_AUTO, s);
+    return e->gather(src, workspace, dst, n, TROVE_VARIANT_AUTO, 0, s);
 }
 
 int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, int tile, void* stream) {

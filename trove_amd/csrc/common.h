@@ -181,9 +181,9 @@ inline int word_bytes_of(int b) { return b % 16 == 0 ? 16 : b % 8 == 0 ? 8 : b %
 template <int B>
 int copy_impl(const void* src, void* dst, int64_t n, int variant, int tile, hipStream_t s);
 template <int B>
-int gather_impl(const void* src, const int32_t* idx, void* dst, int64_t n, int variant, hipStream_t s);
+int gather_impl(const void* src, const int32_t* idx, void* dst, int64_t n, int variant, int tile, hipStream_t s);
 template <int B>
-int scatter_impl(const void* src, const int32_t* idx, void* dst, int64_t n, int variant, hipStream_t s);
+int scatter_impl(const void* src, const int32_t* idx, void* dst, int64_t n, int variant, int tile, hipStream_t s);
 template <int B>
 int counting_impl(void* dst, int64_t n, int variant, int tile, hipStream_t s);
 template <int B>

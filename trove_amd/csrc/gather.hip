@@ -53,6 +53,48 @@ __global__ __launch_bounds__(kBlock) void k_gather_shfl(const T* __restrict__ sr
     if (g < ((n + 63) >> 6)) gather_group_shfl<T>(src, idx, dst, n, g);
 }
 
+// coalesced_ptr gather / scatter at a fixed tile width below 64 (the C ABI's
+// `tile` argument): one group per wavefront, each TILE-lane tile on its own.
+template <typename T, int TILE>
+__global__ __launch_bounds__(kBlock) void k_gather_shfl_tile(const T* __restrict__ src,
+                                                             const int32_t* __restrict__ idx, T* __restrict__ dst,
+                                                             int64_t n) {
+    const int64_t g = wave_index();
+    if (g < ((n + 63) >> 6)) gather_group_shfl_tile<T, TILE>(src, idx, dst, n, g);
+}
+
+template <typename T, int TILE>
+__global__ __launch_bounds__(kBlock) void k_scatter_shfl_tile(const T* __restrict__ src,
+                                                              const int32_t* __restrict__ idx, T* __restrict__ dst,
+                                                              int64_t n) {
+    const int64_t i = (wave_index() << 6) + trove::lane_id();
+    if (i < n) move_at_tile<TILE>(src + i, dst + idx[i]);
+}
+
+// Launches the fixed-tile gather (SCATTER false) or scatter (true).
+template <typename T, bool SCATTER>
+static int launch_shfl_tile(const T* src, const int32_t* idx, T* dst, int64_t n, int tile, hipStream_t s) {
+    const unsigned blocks = (unsigned)grid_for((n + 63) / 64);
+#define TROVE_TILE_CASE(W)                                                                                 \
+    case W:                                                                                                \
+        if constexpr (SCATTER) {                                                                           \
+            hipLaunchKernelGGL((k_scatter_shfl_tile<T, W>), dim3(blocks), dim3(kBlock), 0, s, src, idx, dst, n); \
+        } else {                                                                                           \
+            hipLaunchKernelGGL((k_gather_shfl_tile<T, W>), dim3(blocks), dim3(kBlock), 0, s, src, idx, dst, n);  \
+        }                                                                                                  \
+        break;
+    switch (tile) {
+        TROVE_TILE_CASE(32)
+        TROVE_TILE_CASE(16)
+        TROVE_TILE_CASE(8)
+        TROVE_TILE_CASE(4)
+        TROVE_TILE_CASE(2)
+        default: return TROVE_ERR_VARIANT;
+    }
+#undef TROVE_TILE_CASE
+    return launch_status();
+}
+
 // LDS-staged gather: one group per one-wavefront work-group (64 * sizeof(T)
 // bytes of dynamic LDS, up to 64 KiB at 1 KiB structs); the trailing partial
 // group per lane.
@@ -190,13 +232,18 @@ constexpr bool tile_path() {
 }
 
 template <int B>
-int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int variant, hipStream_t s) {
+int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int variant, int tile, hipStream_t s) {
     typedef blob_t<B> T;
     const T* src = static_cast<const T*>(src_);
     T* dst = static_cast<T*>(dst_);
     constexpr bool tiled = tile_path<T>();
     typedef gather_tile_shape<T> G;
     const int64_t units = ((n + 63) / 64 + G::unroll - 1) / G::unroll;
+    if (tile != 0 && tile != 64) {  // a sub-wavefront tile asks for coalesced_ptr at that width
+        if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL)
+            return launch_shfl_tile<T, false>(src, idx, dst, n, tile, s);
+        if (variant != TROVE_VARIANT_DIRECT) return TROVE_ERR_VARIANT;
+    }
     if (variant == TROVE_VARIANT_AUTO) variant = auto_gather_variant(B);
     if (variant == TROVE_VARIANT_TILE && !tiled) variant = TROVE_VARIANT_DIRECT;
     if (variant == TROVE_VARIANT_LDS) {
@@ -235,11 +282,16 @@ int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int
 }
 
 template <int B>
-int scatter_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int variant, hipStream_t s) {
+int scatter_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int variant, int tile, hipStream_t s) {
     typedef blob_t<B> T;
     const T* src = static_cast<const T*>(src_);
     T* dst = static_cast<T*>(dst_);
     constexpr bool tiled = tile_path<T>();
+    if (tile != 0 && tile != 64) {  // a sub-wavefront tile asks for coalesced_ptr at that width
+        if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL)
+            return launch_shfl_tile<T, true>(src, idx, dst, n, tile, s);
+        if (variant != TROVE_VARIANT_DIRECT) return TROVE_ERR_VARIANT;
+    }
     // 4-byte rows: one group per wavefront. With four, the streaming loads
     // of the next groups slow the random 4-byte stores to half the rate
     // (0.51 vs 0.98 TB/s on MI355X); 8- and 16-byte rows gain from four.
@@ -282,8 +334,8 @@ int scatter_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, in
 }
 
 #define TROVE_INST_GATHER(B)                                                                          \
-    template int gather_impl<B>(const void*, const int32_t*, void*, int64_t, int, hipStream_t);       \
-    template int scatter_impl<B>(const void*, const int32_t*, void*, int64_t, int, hipStream_t);
+    template int gather_impl<B>(const void*, const int32_t*, void*, int64_t, int, int, hipStream_t);  \
+    template int scatter_impl<B>(const void*, const int32_t*, void*, int64_t, int, int, hipStream_t);
 TROVE_FOR_EACH_SIZE_IN_SHARD(TROVE_INST_GATHER)
 
 }  // namespace trove_amd

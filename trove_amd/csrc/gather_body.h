@@ -144,6 +144,31 @@ __device__ __forceinline__ void gather_group_shfl(const T* __restrict__ src, con
     guard.report();
 }
 
+// coalesced_ptr path at a fixed tile width TILE < 64: each converged
+// TILE-lane tile of the wavefront moves its own structs (what the
+// reference's coalesced_ptr does on a 32-lane warp); a tile with inactive
+// lanes (in a partial last group) moves its structs per lane. The load and
+// the store see the same active lanes, so one convergence check serves both,
+// and each branch keeps its struct in registers (a struct returned through a
+// branch landed in scratch memory at 512 bytes and above).
+template <int TILE, typename T>
+__device__ __forceinline__ void move_at_tile(const T* from, T* to) {
+    if (trove::warp_converged<TILE>()) {
+        const trove::thread_tile<TILE> tile;
+        const T v = trove::detail::load_dispatch(from, tile);
+        trove::detail::store_dispatch(v, to, tile);
+    } else {
+        trove::detail::divergent_store(trove::detail::divergent_load(from), to);
+    }
+}
+
+template <typename T, int TILE>
+__device__ __forceinline__ void gather_group_shfl_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                       T* __restrict__ dst, int64_t n, int64_t g) {
+    const int64_t i = (g << 6) + trove::lane_id();
+    if (i < n) move_at_tile<TILE>(src + idx[i], dst + i);
+}
+
 // LDS-staged path (trove/lds.h) for ONE full group g, the LDS counterpart of
 // the coalesced_ptr path: the tile path's indexed loads (column-side words)
 // go through the stage so that lane l holds struct l, as a coalesced_ptr read
