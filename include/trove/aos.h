@@ -154,7 +154,7 @@ __host__ __device__ __forceinline__ void word_owners(int id, int (&owner)[m], in
 // Word access j goes to base + off[j] (in structs), word word[j]; off[j] is
 // the 32-bit value `key` of lane first + owner[j]. Broadcasts first, then
 // accesses, chunk by chunk.
-template <int n, typename T, bool kStore, typename V>
+template <int n, typename T, bool kStore, typename Policy = cached, typename V>
 __device__ __forceinline__ void tile_words_by_key(T* base, int key, int id, int first, V* v) {
     typedef word_layout<T> L;
     typedef typename std::conditional<std::is_const<T>::value, const typename L::U, typename L::U>::type U;
@@ -175,9 +175,9 @@ __device__ __forceinline__ void tile_words_by_key(T* base, int key, int id, int 
             if constexpr (kStore) {
                 typename L::U u;
                 __builtin_memcpy(&u, &v[(j0 + j) * L::W], sizeof(u));
-                store_word<cached>(p, u);
+                store_word<Policy>(p, u);
             } else {
-                const typename L::U u = load_word<cached>(p);
+                const typename L::U u = load_word<Policy>(p);
                 __builtin_memcpy(&v[(j0 + j) * L::W], &u, sizeof(u));
             }
         }
@@ -186,7 +186,7 @@ __device__ __forceinline__ void tile_words_by_key(T* base, int key, int id, int 
 
 // Same with full per-lane pointers (owner offsets beyond 32 bits): each word
 // receives its owner's pointer.
-template <int n, typename T, bool kStore, typename V>
+template <int n, typename T, bool kStore, typename Policy = cached, typename V>
 __device__ __forceinline__ void tile_words_by_pointer(T* own, int id, int first, V* v) {
     typedef word_layout<T> L;
     typedef typename std::conditional<std::is_const<T>::value, const typename L::U, typename L::U>::type U;
@@ -208,9 +208,9 @@ __device__ __forceinline__ void tile_words_by_pointer(T* own, int id, int first,
             if constexpr (kStore) {
                 typename L::U u;
                 __builtin_memcpy(&u, &v[(j0 + j) * L::W], sizeof(u));
-                store_word<cached>(p, u);
+                store_word<Policy>(p, u);
             } else {
-                const typename L::U u = load_word<cached>(p);
+                const typename L::U u = load_word<Policy>(p);
                 __builtin_memcpy(&v[(j0 + j) * L::W], &u, sizeof(u));
             }
         }
@@ -238,41 +238,41 @@ __device__ __forceinline__ tile_offset offset_from(const T* p, const T* p0) {
 // Loads the words of a tile whose structs sit at arbitrary per-lane
 // addresses, in the column-side layout (word j of rank id = tile word
 // id + j*n of the concatenated structs).
-template <int n, typename T>
+template <int n, typename Policy = cached, typename T>
 __device__ __forceinline__ void tile_gather_words(const T* src, const T* p0, tile_offset o, int id, int first,
                                                   uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
     if (tile_all<n>(o.exact && o.delta == (long long)(int)o.delta)) {
-        tile_words_by_key<n, const T, false>(p0, (int)o.delta, id, first, v);
+        tile_words_by_key<n, const T, false, Policy>(p0, (int)o.delta, id, first, v);
     } else {
-        tile_words_by_pointer<n, const T, false>(src, id, first, v);
+        tile_words_by_pointer<n, const T, false, Policy>(src, id, first, v);
     }
 }
 
-template <int n, typename T>
+template <int n, typename Policy = cached, typename T>
 __device__ __forceinline__ void tile_scatter_words(T* dst, T* p0, tile_offset o, int id, int first,
                                                    const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
     if (tile_all<n>(o.exact && o.delta == (long long)(int)o.delta)) {
-        tile_words_by_key<n, T, true>(p0, (int)o.delta, id, first, v);
+        tile_words_by_key<n, T, true, Policy>(p0, (int)o.delta, id, first, v);
     } else {
-        tile_words_by_pointer<n, T, true>(dst, id, first, v);
+        tile_words_by_pointer<n, T, true, Policy>(dst, id, first, v);
     }
 }
 
 // Tile gather from one array: rank id wants base[index]. Produces the
 // column-side layout of the gathered structs (what a coalesced tile store
 // expects), broadcasting the 32-bit index of each word's owner.
-template <int n, typename T>
+template <int n, typename Policy = cached, typename T>
 __device__ __forceinline__ void tile_gather_indexed(const T* base, int index, int id, int first,
                                                     uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
-    tile_words_by_key<n, const T, false>(base, index, id, first, v);
+    tile_words_by_key<n, const T, false, Policy>(base, index, id, first, v);
 }
 
 // Tile scatter into one array: the structs held in column-side layout go to
 // base[index of their owner rank].
-template <int n, typename T>
+template <int n, typename Policy = cached, typename T>
 __device__ __forceinline__ void tile_scatter_indexed(T* base, int index, int id, int first,
                                                      const uint32_t (&v)[word_layout<T>::m * word_layout<T>::W]) {
-    tile_words_by_key<n, T, true>(base, index, id, first, v);
+    tile_words_by_key<n, T, true, Policy>(base, index, id, first, v);
 }
 
 }  // namespace detail
@@ -339,7 +339,7 @@ __device__ __forceinline__ typename enable_if<detail::use_direct<T, tile_size>::
 
 namespace detail {
 
-template <typename T, typename Tile>
+template <typename Policy = cached, typename T, typename Tile>
 __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value, T>::type load_dispatch(const T* src,
                                                                                        const Tile& tile) {
     constexpr int n = Tile::size();
@@ -350,9 +350,9 @@ __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value, 
     const tile_offset o = offset_from(src, p0);
     uint32_t v[L::m * L::W];
     if (tile_all<n>(o.exact && o.delta == id)) {
-        tile_load_words<n>(p0, id, v);
+        tile_load_words<n, T, Policy>(p0, id, v);
     } else {
-        tile_gather_words<n>(src, p0, o, id, first, v);
+        tile_gather_words<n, Policy>(src, p0, o, id, first, v);
     }
     r2c_words<n, L::m, L::W>(v, id, first);
     T out;
@@ -360,12 +360,12 @@ __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value, 
     return out;
 }
 
-template <typename T, typename Tile>
+template <typename Policy = cached, typename T, typename Tile>
 __device__ __forceinline__ typename enable_if<use_direct<T, Tile::size()>::value, T>::type load_dispatch(const T* src, const Tile&) {
-    return divergent_load(src);
+    return divergent_load<Policy>(src);
 }
 
-template <typename T, typename Tile>
+template <typename Policy = cached, typename T, typename Tile>
 __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value>::type store_dispatch(const T& data, T* dest,
                                                                                      const Tile& tile) {
     constexpr int n = Tile::size();
@@ -378,57 +378,59 @@ __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value>:
     struct_to_words(data, v);
     c2r_words<n, L::m, L::W>(v, id, first);
     if (tile_all<n>(o.exact && o.delta == id)) {
-        tile_store_words<n>(p0, id, v);
+        tile_store_words<n, T, Policy>(p0, id, v);
     } else {
-        tile_scatter_words<n>(dest, p0, o, id, first, v);
+        tile_scatter_words<n, Policy>(dest, p0, o, id, first, v);
     }
 }
 
-template <typename T, typename Tile>
+template <typename Policy = cached, typename T, typename Tile>
 __device__ __forceinline__ typename enable_if<use_direct<T, Tile::size()>::value>::type store_dispatch(const T& data, T* dest,
                                                                                        const Tile&) {
-    divergent_store(data, dest);
+    divergent_store<Policy>(data, dest);
 }
 
 }  // namespace detail
 
 // Load *src cooperatively with the widest converged tile around the lane.
-template <typename T>
+// Policy (an addition, defaulted to the reference's behaviour) is the cache
+// policy of the global accesses: trove::cached or trove::streaming.
+template <typename T, typename Policy = cached>
 __device__ __forceinline__ T load(const T* src) {
     if (warp_converged<WARP_SIZE>()) {
-        return detail::load_dispatch(src, thread_tile<WARP_SIZE>());
+        return detail::load_dispatch<Policy>(src, thread_tile<WARP_SIZE>());
     } else if (warp_converged<WARP_SIZE / 2>()) {
-        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 2>());
+        return detail::load_dispatch<Policy>(src, thread_tile<WARP_SIZE / 2>());
     } else if (warp_converged<WARP_SIZE / 4>()) {
-        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 4>());
+        return detail::load_dispatch<Policy>(src, thread_tile<WARP_SIZE / 4>());
     } else if (warp_converged<WARP_SIZE / 8>()) {
-        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 8>());
+        return detail::load_dispatch<Policy>(src, thread_tile<WARP_SIZE / 8>());
     } else if (warp_converged<WARP_SIZE / 16>()) {
-        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 16>());
+        return detail::load_dispatch<Policy>(src, thread_tile<WARP_SIZE / 16>());
     } else if (warp_converged<WARP_SIZE / 32>()) {
-        return detail::load_dispatch(src, thread_tile<WARP_SIZE / 32>());
+        return detail::load_dispatch<Policy>(src, thread_tile<WARP_SIZE / 32>());
     } else {
-        return detail::divergent_load(src);
+        return detail::divergent_load<Policy>(src);
     }
 }
 
 // Store data to *dest cooperatively with the widest converged tile.
-template <typename T>
+template <typename T, typename Policy = cached>
 __device__ __forceinline__ void store(const T& data, T* dest) {
     if (warp_converged<WARP_SIZE>()) {
-        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE>());
+        detail::store_dispatch<Policy>(data, dest, thread_tile<WARP_SIZE>());
     } else if (warp_converged<WARP_SIZE / 2>()) {
-        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 2>());
+        detail::store_dispatch<Policy>(data, dest, thread_tile<WARP_SIZE / 2>());
     } else if (warp_converged<WARP_SIZE / 4>()) {
-        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 4>());
+        detail::store_dispatch<Policy>(data, dest, thread_tile<WARP_SIZE / 4>());
     } else if (warp_converged<WARP_SIZE / 8>()) {
-        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 8>());
+        detail::store_dispatch<Policy>(data, dest, thread_tile<WARP_SIZE / 8>());
     } else if (warp_converged<WARP_SIZE / 16>()) {
-        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 16>());
+        detail::store_dispatch<Policy>(data, dest, thread_tile<WARP_SIZE / 16>());
     } else if (warp_converged<WARP_SIZE / 32>()) {
-        detail::store_dispatch(data, dest, thread_tile<WARP_SIZE / 32>());
+        detail::store_dispatch<Policy>(data, dest, thread_tile<WARP_SIZE / 32>());
     } else {
-        detail::divergent_store(data, dest);
+        detail::divergent_store<Policy>(data, dest);
     }
 }
 

@@ -3,7 +3,8 @@
 //
 // Contract of trove/detail/fallback.h:67-129: when sizeof(T) is a multiple of
 // 4 the struct moves as its m words of type U (dismember_type), each lane on
-// its own; otherwise it is an ordinary compiler-generated access.
+// its own, with the cache policy given (cached by default); otherwise it is
+// an ordinary compiler-generated access.
 #pragma once
 #include <trove/detail/dismember.h>
 
@@ -18,7 +19,7 @@ struct use_divergent {
 // Each word is copied into (out of) the struct on its own: one memcpy of a
 // whole struct of several hundred bytes is not split into registers and
 // lands in scratch memory.
-template <typename T>
+template <typename Policy = cached, typename T>
 __device__ __forceinline__ typename enable_if<use_divergent<T>::value, T>::type divergent_load(const T* src) {
     typedef typename dismember_type<T>::type U;
     constexpr int m = aliased_size<T, U>::value;
@@ -26,18 +27,18 @@ __device__ __forceinline__ typename enable_if<use_divergent<T>::value, T>::type 
     T out;
 #pragma unroll
     for (int j = 0; j < m; ++j) {
-        const U u = p[j];
+        const U u = load_word<Policy>(p + j);
         __builtin_memcpy(reinterpret_cast<char*>(&out) + j * sizeof(U), &u, sizeof(U));
     }
     return out;
 }
 
-template <typename T>
+template <typename Policy = cached, typename T>
 __device__ __forceinline__ typename enable_if<!use_divergent<T>::value, T>::type divergent_load(const T* src) {
     return *src;
 }
 
-template <typename T>
+template <typename Policy = cached, typename T>
 __device__ __forceinline__ typename enable_if<use_divergent<T>::value>::type divergent_store(const T& data, T* dest) {
     typedef typename dismember_type<T>::type U;
     constexpr int m = aliased_size<T, U>::value;
@@ -46,11 +47,11 @@ __device__ __forceinline__ typename enable_if<use_divergent<T>::value>::type div
     for (int j = 0; j < m; ++j) {
         U u;
         __builtin_memcpy(&u, reinterpret_cast<const char*>(&data) + j * sizeof(U), sizeof(U));
-        p[j] = u;
+        store_word<Policy>(p + j, u);
     }
 }
 
-template <typename T>
+template <typename Policy = cached, typename T>
 __device__ __forceinline__ typename enable_if<!use_divergent<T>::value>::type divergent_store(const T& data, T* dest) {
     *dest = data;
 }

@@ -12,35 +12,46 @@
 namespace trove {
 namespace detail {
 
-template <typename T>
+template <typename T, typename Policy = cached>
 struct coalesced_ref {
     T* m_ptr;
     __device__ explicit coalesced_ref(T* ptr) : m_ptr(ptr) {}
 
-    __device__ __forceinline__ operator T() { return trove::load(m_ptr); }
+    __device__ __forceinline__ operator T() { return trove::load<T, Policy>(m_ptr); }
 
     __device__ __forceinline__ coalesced_ref& operator=(const T& data) {
-        trove::store(data, m_ptr);
+        trove::store<T, Policy>(data, m_ptr);
         return *this;
     }
 
     __device__ __forceinline__ coalesced_ref& operator=(const coalesced_ref& other) {
-        const T data = trove::load(other.m_ptr);
-        trove::store(data, m_ptr);
+        const T data = trove::load<T, Policy>(other.m_ptr);
+        trove::store<T, Policy>(data, m_ptr);
+        return *this;
+    }
+
+    // From a reference of the other policy: each side keeps its own.
+    template <typename P, typename = typename std::enable_if<!std::is_same<P, Policy>::value>::type>
+    __device__ __forceinline__ coalesced_ref& operator=(const coalesced_ref<T, P>& other) {
+        const T data = trove::load<T, P>(other.m_ptr);
+        trove::store<T, Policy>(data, m_ptr);
         return *this;
     }
 };
 
 }  // namespace detail
 
-template <typename T>
+// Policy (an addition, defaulted to the reference's behaviour) is the cache
+// policy of the element accesses: trove::cached, or trove::streaming for
+// arrays touched once (nontemporal loads and stores).
+template <typename T, typename Policy = cached>
 struct coalesced_ptr {
     T* m_ptr;
     __device__ coalesced_ptr(T* ptr) : m_ptr(ptr) {}
-    __device__ trove::detail::coalesced_ref<T> operator*() { return trove::detail::coalesced_ref<T>(m_ptr); }
+    __device__ trove::detail::coalesced_ref<T, Policy> operator*() { return trove::detail::coalesced_ref<T, Policy>(m_ptr); }
     template <typename I>
-    __device__ trove::detail::coalesced_ref<T> operator[](const I& idx) {
-        return trove::detail::coalesced_ref<T>(m_ptr + idx);
+    __device__ trove::detail::coalesced_ref<T, Policy> operator[](const I& idx) {
+        return trove::detail::coalesced_ref<T, Policy>(m_ptr + idx);
     }
     __device__ operator T*() { return m_ptr; }
 };

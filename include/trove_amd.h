@@ -90,7 +90,7 @@ int trove_copy(const void* src, void* dst, int64_t n, int struct_bytes, int vari
  * stores with the transposition elided (4-, 8- and 16-byte structs: per-lane
  * row words); LDS: the tile-indexed loads staged through LDS to one struct
  * per lane and back out with dwordx4 stores (the LDS-staged counterpart of
- * SHFL). AUTO: LDS for 16, 48, 64 and 192 bytes, else TILE.
+ * SHFL). AUTO: LDS for 16, 48, 64, 192 and 384 bytes, SHFL for 224, else TILE.
  * tile: 0 or 64 = whole wavefronts; a power of two 2..32 runs SHFL (also for
  * AUTO) with each converged tile of that many lanes on its own, as the
  * reference's coalesced_ptr runs on a 32-lane warp; TILE and LDS need whole

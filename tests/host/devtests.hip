@@ -129,6 +129,33 @@ __global__ __launch_bounds__(64) void k_cascade(uint64_t mask, int scatter, cons
     }
 }
 
+// The same through the streaming cache policy: mode 0 calls
+// trove::load / trove::store<T, streaming>, mode 1 assigns between two
+// streaming coalesced_ptrs, mode 2 assigns a cached coalesced_ptr's element
+// to a streaming one (the cross-policy ref = ref).
+template <int S>
+__global__ __launch_bounds__(64) void k_cascade_policy(uint64_t mask, int scatter, int mode,
+                                                       const trove::array<uint32_t, S / 4>* src, const int* perm,
+                                                       trove::array<uint32_t, S / 4>* dst) {
+    typedef trove::array<uint32_t, S / 4> T;
+    typedef trove::streaming NT;
+    const int lane = threadIdx.x;
+    if ((mask >> lane) & 1ull) {
+        const int from = scatter ? lane : perm[lane], to = scatter ? perm[lane] : lane;
+        if (mode == 0) {
+            trove::store<T, NT>(trove::load<T, NT>(src + from), dst + to);
+        } else if (mode == 1) {
+            trove::coalesced_ptr<T, NT> s(const_cast<T*>(src));
+            trove::coalesced_ptr<T, NT> d(dst);
+            d[to] = s[from];
+        } else {
+            trove::coalesced_ptr<T> s(const_cast<T*>(src));
+            trove::coalesced_ptr<T, NT> d(dst);
+            d[to] = s[from];
+        }
+    }
+}
+
 // coalesced_ptr over a __shared__ array: active lanes (mask) read struct
 // perm[lane] of 64 structs staged in LDS, through the dynamic path (pointers
 // into LDS are flat pointers to the LDS aperture).
@@ -292,6 +319,18 @@ int devtest_cascade(int s, uint64_t mask, int scatter, const void* src, const in
     }
     CS(8) CS(12) CS(20) CS(32) CS(48) CS(60) CS(96) CS(384) CS(512) CS(1024)
 #undef CS
+    return -1;
+}
+
+int devtest_cascade_policy(int s, uint64_t mask, int scatter, int mode, const void* src, const int* perm, void* dst) {
+#define CP(S)                                                                                        \
+    if (s == S) {                                                                                    \
+        hipLaunchKernelGGL((k_cascade_policy<S>), dim3(1), dim3(64), 0, 0, mask, scatter, mode,     \
+                           (const trove::array<uint32_t, S / 4>*)src, perm, (trove::array<uint32_t, S / 4>*)dst); \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                                        \
+    }
+    CP(8) CP(20) CP(32) CP(60) CP(64) CP(96) CP(256)
+#undef CP
     return -1;
 }
 

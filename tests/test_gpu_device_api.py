@@ -44,6 +44,8 @@ def dt(cuda):
     L.devtest_converged.argtypes = [u64, vp]
     L.devtest_shfl.argtypes = [i, vp, vp]
     L.devtest_cascade.argtypes = [i, u64, i, vp, vp, vp]
+    L.devtest_cascade_policy.argtypes = [i, u64, i, i, vp, vp, vp]
+    L.devtest_cascade_policy.restype = i
     L.devtest_shared_gather.argtypes = [i, u64, vp, vp, vp]
     L.devtest_shared_gather.restype = i
     L.devtest_mixed.argtypes = [i, u64, vp, vp, vp, vp]
@@ -181,6 +183,35 @@ def test_cascade_under_masks(dt, s):
                         else:
                             want[lane] = src2[perm[lane]]
                 assert np.array_equal(got, want), (s, hex(mask), scatter)
+
+
+@pytest.mark.parametrize("s", [8, 20, 32, 60, 64, 96, 256])
+def test_streaming_policy_cascade(dt, s):
+    # The cascade with nontemporal accesses (trove::streaming) moves the same
+    # structs as the cached one: trove::load / store<T, streaming>, two
+    # streaming coalesced_ptrs, and a cached source assigned to a streaming
+    # destination; gathers and scatters under arbitrary lane masks.
+    rng = np.random.default_rng(900 + s)
+    k = s // 4
+    for mask in _masks()[:10]:
+        for scatter in (0, 1):
+            for mode in (0, 1, 2):
+                perm = rng.permutation(64).astype(np.int32)
+                src = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
+                init = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
+                d_dst = torch.from_numpy(init.view(np.int32)).cuda()
+                assert dt.devtest_cascade_policy(s, mask, scatter, mode, torch.from_numpy(src.view(np.int32)).cuda().data_ptr(),
+                                                 torch.from_numpy(perm).cuda().data_ptr(), d_dst.data_ptr()) == 0
+                got = d_dst.cpu().numpy().view(np.uint32).reshape(64, k)
+                want = init.reshape(64, k).copy()
+                src2 = src.reshape(64, k)
+                for lane in range(64):
+                    if (mask >> lane) & 1:
+                        if scatter:
+                            want[perm[lane]] = src2[lane]
+                        else:
+                            want[lane] = src2[perm[lane]]
+                assert np.array_equal(got, want), (s, hex(mask), scatter, mode)
 
 
 @pytest.mark.parametrize("s", [8, 20, 32, 60, 64])

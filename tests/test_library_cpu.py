@@ -86,8 +86,8 @@ def test_auto_variant_table():
     # What AUTO resolves to, per op and size (no GPU needed).
     assert [T.auto_variant("copy", b) for b in (4, 8, 12, 16, 20, 32, 60, 96, 128, 160, 256, 1024, 6)] == \
         ["lds", "lds", "lds", "lds", "shfl", "lds", "shfl", "lds", "lds", "lds", "shfl", "shfl", "shfl"]
-    assert [T.auto_variant("gather", b) for b in (4, 16, 20, 48, 64, 192, 512, 2)] == \
-        ["tile", "lds", "tile", "lds", "lds", "lds", "tile", "direct"]
+    assert [T.auto_variant("gather", b) for b in (4, 16, 20, 48, 64, 192, 224, 384, 512, 2)] == \
+        ["tile", "lds", "tile", "lds", "lds", "lds", "shfl", "lds", "tile", "direct"]
     assert [T.auto_variant("scatter", b) for b in (4, 8, 16, 20, 64, 6)] == \
         ["direct", "lds", "lds", "tile", "tile", "direct"]
     assert T.lib().trove_auto_variant(3, 20) == T.ERR_INVALID

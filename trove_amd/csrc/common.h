@@ -20,12 +20,14 @@ constexpr int kBlock = 256;
 constexpr int kMaxLdsStructBytes = 512;
 
 // TROVE_VARIANT_AUTO per struct size, from interleaved A/B timings on MI355X
-// (median of 15 launches, 2^26 structs up to 64 bytes, 2^24 up to 256, 2^22
-// above) confirmed by a second, independent bench run: the LDS-staged path
-// only where it was faster in both. Copies: the structs of at most 16 bytes
-// (a full dwordx4 per lane instead of narrower per-lane words) and the
-// 16-byte-word structs of 32 and 96-160 bytes (no two-rotation schedule);
-// gathers: 16, 48, 64 and 192 bytes; scatters: 8 and 16 bytes. 4-byte
+// (median of 9-15 launches, 2^26 structs up to 64 bytes, 2^24-2^25 up to
+// 256, 2^22-2^23 above) confirmed by a second, independent bench run: a path
+// other than the default only where it was faster in both. Copies: the
+// LDS-staged path for structs of at most 16 bytes (a full dwordx4 per lane
+// instead of narrower per-lane words) and the 16-byte-word structs of 32 and
+// 96-160 bytes (no two-rotation schedule); gathers: LDS-staged at 16, 48,
+// 64, 192 and 384 bytes (+6-9% over the tile path at 384), coalesced_ptr at
+// 224 bytes (+6-7%); scatters: LDS-staged at 8 and 16 bytes. 4-byte
 // scatters keep the naive kernel (0.93 TB/s; the streaming one-word tile
 // scatter reaches 0.84). Everything else takes the transposition (copy) or
 // the tile path (gather, scatter).
@@ -35,7 +37,9 @@ constexpr int auto_copy_variant(int b) {
                : TROVE_VARIANT_SHFL;
 }
 constexpr int auto_gather_variant(int b) {
-    return (b == 16 || b == 48 || b == 64 || b == 192) ? TROVE_VARIANT_LDS : TROVE_VARIANT_TILE;
+    return (b == 16 || b == 48 || b == 64 || b == 192 || b == 384) ? TROVE_VARIANT_LDS
+           : b == 224                                                ? TROVE_VARIANT_SHFL
+                                                                     : TROVE_VARIANT_TILE;
 }
 constexpr int auto_scatter_variant(int b) {
     return b == 4 ? TROVE_VARIANT_DIRECT : (b == 8 || b == 16) ? TROVE_VARIANT_LDS : TROVE_VARIANT_TILE;

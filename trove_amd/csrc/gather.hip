@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_shfl_tile(const T* __restric
                                                               const int32_t* __restrict__ idx, T* __restrict__ dst,
                                                               int64_t n) {
     const int64_t i = (wave_index() << 6) + trove::lane_id();
-    if (i < n) move_at_tile<TILE>(src + i, dst + idx[i]);
+    if (i < n) move_at_tile<TILE, trove::cached, trove::cached>(src + i, dst + idx[i]);
 }
 
 // Launches the fixed-tile gather (SCATTER false) or scatter (true).

@@ -37,6 +37,14 @@ struct row_guard {
     }
 };
 
+// Cache policy of the random row reads of every gather path: ordinary loads
+// up to 48 bytes, nontemporal loads above. Measured on MI355X (2^26 random
+// rows, interleaved runs): nontemporal row loads cost 10-20% at 8-24 bytes,
+// are even from 28 to 48 bytes, and gain 1-4% from 52 to 64 bytes and 2-17%
+// from 96 to 256 bytes, on the tile path and through coalesced_ptr alike.
+template <typename T>
+using row_read_policy = typename std::conditional<(sizeof(T) > 48), trove::streaming, trove::cached>::type;
+
 // 64-struct groups a wavefront keeps in flight on the tile path. Random rows
 // cost whole 128-byte line fills, and the HBM serves random lines faster the
 // more of them are outstanding (measured on MI355X: 3.8 TB/s of line reads
@@ -53,8 +61,7 @@ constexpr int gather_groups_in_flight() {
 // transposition. UNROLL groups are in flight per iteration. Structs of 4, 8
 // or 16 bytes take gather_range_word.
 // The index stream and the destination are touched once (streaming policy);
-// the gathered source keeps ordinary loads (nontemporal loads of random rows
-// measured 20-30% slower on MI355X).
+// the gathered rows follow row_read_policy.
 // One-word structs (m = 1: 4, 8 and 16 bytes) have nothing to broadcast or
 // transpose: each lane loads its row as one word. They keep the tile path's
 // policies (streaming index loads and stores, UNROLL groups in flight).
@@ -111,13 +118,14 @@ __device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, con
         for (int u = 0; u < UNROLL; ++u) ix[u] = guard(trove::detail::load_word<S>(idx + ((g + u) << 6) + lane));
         uint32_t v[UNROLL][L::m * L::W];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) trove::detail::tile_gather_indexed<64>(src, ix[u], lane, 0, v[u]);
+        for (int u = 0; u < UNROLL; ++u) trove::detail::tile_gather_indexed<64, row_read_policy<T>>(src, ix[u], lane, 0, v[u]);
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) trove::detail::tile_store_words<64, T, S>(dst + ((g + u) << 6), lane, v[u]);
     }
     for (; g < full_end; ++g) {
         uint32_t v[L::m * L::W];
-        trove::detail::tile_gather_indexed<64>(src, guard(trove::detail::load_word<S>(idx + (g << 6) + lane)), lane, 0, v);
+        trove::detail::tile_gather_indexed<64, row_read_policy<T>>(src, guard(trove::detail::load_word<S>(idx + (g << 6) + lane)),
+                                                                   lane, 0, v);
         trove::detail::tile_store_words<64, T, S>(dst + (g << 6), lane, v);
     }
     if (g1 > full && (n & 63)) {
@@ -128,16 +136,19 @@ __device__ __forceinline__ void gather_range_tile(const T* __restrict__ src, con
 }
 
 // coalesced_ptr path (tests/benchmark.cu:90-99) for ONE group g: a partial
-// last group runs the tile cascade with its missing lanes inactive. There is
-// deliberately no loop here: around the cascade, LICM would keep the tables
-// of all six tile levels live at once.
+// last group runs the tile cascade with its missing lanes inactive. The
+// destination is written once, so it is a streaming coalesced_ptr
+// (nontemporal stores: +3-8% over cached ones on MI355X at 12-64 bytes); the
+// rows are read with row_read_policy. There is deliberately no loop here:
+// around the cascade, LICM would keep the tables of all six tile levels live
+// at once.
 template <typename T, typename G = no_guard>
 __device__ __forceinline__ void gather_group_shfl(const T* __restrict__ src, const int32_t* __restrict__ idx,
                                                   T* __restrict__ dst, int64_t n, int64_t g, G guard = G()) {
     const int64_t i = (g << 6) + trove::lane_id();
     if (i < n) {
-        trove::coalesced_ptr<T> s(const_cast<T*>(src));
-        trove::coalesced_ptr<T> r(dst);
+        trove::coalesced_ptr<T, row_read_policy<T>> s(const_cast<T*>(src));
+        trove::coalesced_ptr<T, trove::streaming> r(dst);
         const T v = s[guard(idx[i])];
         r[i] = v;
     }
@@ -150,15 +161,17 @@ __device__ __forceinline__ void gather_group_shfl(const T* __restrict__ src, con
 // lanes (in a partial last group) moves its structs per lane. The load and
 // the store see the same active lanes, so one convergence check serves both,
 // and each branch keeps its struct in registers (a struct returned through a
-// branch landed in scratch memory at 512 bytes and above).
-template <int TILE, typename T>
+// branch landed in scratch memory at 512 bytes and above). LoadP / StoreP:
+// the cache policies of the two sides.
+template <int TILE, typename LoadP, typename StoreP, typename T>
 __device__ __forceinline__ void move_at_tile(const T* from, T* to) {
+    namespace d = trove::detail;
     if (trove::warp_converged<TILE>()) {
         const trove::thread_tile<TILE> tile;
-        const T v = trove::detail::load_dispatch(from, tile);
-        trove::detail::store_dispatch(v, to, tile);
+        const T v = d::load_dispatch<LoadP>(from, tile);
+        d::store_dispatch<StoreP>(v, to, tile);
     } else {
-        trove::detail::divergent_store(trove::detail::divergent_load(from), to);
+        d::divergent_store<StoreP>(d::divergent_load<LoadP>(from), to);
     }
 }
 
@@ -166,7 +179,7 @@ template <typename T, int TILE>
 __device__ __forceinline__ void gather_group_shfl_tile(const T* __restrict__ src, const int32_t* __restrict__ idx,
                                                        T* __restrict__ dst, int64_t n, int64_t g) {
     const int64_t i = (g << 6) + trove::lane_id();
-    if (i < n) move_at_tile<TILE>(src + idx[i], dst + i);
+    if (i < n) move_at_tile<TILE, row_read_policy<T>, trove::streaming>(src + idx[i], dst + i);
 }
 
 // LDS-staged path (trove/lds.h) for ONE full group g, the LDS counterpart of
@@ -187,7 +200,7 @@ __device__ __forceinline__ void gather_group_lds(const T* __restrict__ src, cons
         v = d::divergent_load(src + ix);
     } else {
         uint32_t w[L::m * L::W];
-        d::tile_gather_indexed<64>(src, ix, lane, 0, w);
+        d::tile_gather_indexed<64, row_read_policy<T>>(src, ix, lane, 0, w);
         d::wave_lds_sync();
         d::lds_put_columns<T>(stage, lane, w);
         d::wave_lds_sync();
