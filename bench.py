@@ -10,7 +10,11 @@ for every k
     [0, N), through the tile path (indexed tile loads + coalesced stores) and
     through coalesced_ptr (the Trove high-level API).
 The naive per-lane kernels run on the same inputs outside the timed steps for
-the "vs naive" ratios. Inputs are resident in HBM before timing starts.
+the "vs naive" ratios, twice: as the reference writes them (one struct per
+thread, ordinary accesses, a grid-stride loop) and in the library's own
+launch (DIRECT_LAUNCH: the same grid, cache policies and groups in flight),
+which isolates what the transposition adds. Inputs are resident in HBM
+before timing starts.
 
 Algorithmic bytes (the reference harness's accounting, tests/benchmark.cu:
 147, 210, 283): copy 2*S per struct, gather 2*S + 4 per struct.
@@ -655,6 +659,12 @@ def main():
                 nclock.run(f"copy_direct_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="direct"))
                 nclock.run(f"gather_direct_k{k}", (2 * s + 4) * n,
                            lambda: T.gather(src, idx, dst, n, s, variant="direct"))
+                # The naive per-lane access launched as the library's own
+                # kernels are: what the transposition adds on equal terms.
+                nclock.run(f"copy_direct_launch_k{k}", 2 * s * n,
+                           lambda: T.copy(src, dst, n, s, variant="direct_launch"))
+                nclock.run(f"gather_direct_launch_k{k}", (2 * s + 4) * n,
+                           lambda: T.gather(src, idx, dst, n, s, variant="direct_launch"))
                 # The LDS-staged variant (trove/lds.h) beside the transposition.
                 nclock.run(f"copy_lds_k{k}", 2 * s * n, lambda: T.copy(src, dst, n, s, variant="lds"))
                 nclock.run(f"gather_lds_k{k}", (2 * s + 4) * n, lambda: T.gather(src, idx, dst, n, s, variant="lds"))
@@ -663,6 +673,8 @@ def main():
                 nclock.run(f"scatter_shfl_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="shfl"))
                 nclock.run(f"scatter_direct_k{k}", (2 * s + 4) * n,
                            lambda: T.scatter(src, idx, dst, n, s, variant="direct"))
+                nclock.run(f"scatter_direct_launch_k{k}", (2 * s + 4) * n,
+                           lambda: T.scatter(src, idx, dst, n, s, variant="direct_launch"))
                 nclock.run(f"scatter_lds_k{k}", (2 * s + 4) * n, lambda: T.scatter(src, idx, dst, n, s, variant="lds"))
                 # The permutation scatter (inversion into ws + gather where
                 # that is faster; trove_scatter_permutation).
@@ -736,6 +748,13 @@ def main():
             row.update({"scatter_tile_gbs": naive[f"scatter_tile_k{k}"]["gbs"],
                         "scatter_coalesced_ptr_gbs": naive[f"scatter_shfl_k{k}"]["gbs"],
                         "scatter_naive_gbs": naive[f"scatter_direct_k{k}"]["gbs"]})
+            # The naive access in the library's launch (DIRECT_LAUNCH).
+            cl, gl = naive[f"copy_direct_launch_k{k}"]["gbs"], naive[f"gather_direct_launch_k{k}"]["gbs"]
+            row["naive_launch_equal"] = {
+                "copy_gbs": cl, "gather_gbs": gl, "scatter_gbs": naive[f"scatter_direct_launch_k{k}"]["gbs"],
+                "copy_vs": round(row["copy_gbs"] / cl, 2),
+                "gather_tile_vs": round(row["gather_tile_gbs"] / gl, 2),
+                "scatter_tile_vs": round(row["scatter_tile_gbs"] / naive[f"scatter_direct_launch_k{k}"]["gbs"], 2)}
             best_scatter = max(row["scatter_tile_gbs"], row["scatter_coalesced_ptr_gbs"], row["scatter_naive_gbs"],
                                naive[f"scatter_lds_k{k}"]["gbs"])
             row["scatter_permutation"] = {"gbs": naive[f"scatter_perm_k{k}"]["gbs"],
@@ -771,12 +790,16 @@ def main():
                 for _ in range(r3):
                     lclock.run(f"copy_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="shfl"))
                     lclock.run(f"copy_naive_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="direct"))
+                    lclock.run(f"copy_naive_launch_{sb}", 2 * sb * nl,
+                               lambda: T.copy(src, dst, nl, sb, variant="direct_launch"))
                     lclock.run(f"gather_{sb}", (2 * sb + 4) * nl,
                                lambda: T.gather(src, idx_l, dst, nl, sb, variant="tile"))
                     lclock.run(f"gather_coalesced_ptr_{sb}", (2 * sb + 4) * nl,
                                lambda: T.gather(src, idx_l, dst, nl, sb, variant="shfl"))
                     lclock.run(f"gather_naive_{sb}", (2 * sb + 4) * nl,
                                lambda: T.gather(src, idx_l, dst, nl, sb, variant="direct"))
+                    lclock.run(f"gather_naive_launch_{sb}", (2 * sb + 4) * nl,
+                               lambda: T.gather(src, idx_l, dst, nl, sb, variant="direct_launch"))
                     if T.lds_supported(sb):
                         lclock.run(f"copy_lds_{sb}", 2 * sb * nl, lambda: T.copy(src, dst, nl, sb, variant="lds"))
                         lclock.run(f"gather_lds_{sb}", (2 * sb + 4) * nl,

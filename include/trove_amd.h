@@ -52,6 +52,10 @@ extern "C" {
 #define TROVE_VARIANT_TILE 3   /* gather/scatter: tile-indexed word access, transposition elided */
 #define TROVE_VARIANT_LDS 4    /* LDS-staged transposition (trove/lds.h): dwordx4 tile traffic, per-lane LDS
                                   reads; struct sizes that are multiples of 4 up to 512 bytes */
+#define TROVE_VARIANT_DIRECT_LAUNCH 5 /* the naive per-lane struct access launched as SHFL (copy) or TILE
+                                         (gather, scatter) are: the same grid, groups in flight, XCD order
+                                         and cache policies, so that it differs from them only by the
+                                         transposition (a launch-equal baseline; copy, gather, scatter) */
 
 /* Operations for trove_auto_variant. */
 #define TROVE_OP_COPY 0

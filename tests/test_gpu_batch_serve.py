@@ -107,9 +107,10 @@ def test_batch_rejects_bad_jobs(trove, cuda):
     with pytest.raises(T.TroveError) as e:
         T.Batch([(src, idx, src, 16, 68, "tile", 0)])
     assert e.value.code == T.ERR_SIZE
-    with pytest.raises(T.TroveError) as e:
-        T.Batch([(src, idx, src, 16, 32, 9, 0)])
-    assert e.value.code == T.ERR_VARIANT
+    for variant in (9, T.VARIANT_DIRECT_LAUNCH):  # the launch-equal baseline is not a batch variant
+        with pytest.raises(T.TroveError) as e:
+            T.Batch([(src, idx, src, 16, 32, variant, 0)])
+        assert e.value.code == T.ERR_VARIANT
 
 
 def _table(n_rows, row_bytes, seed):

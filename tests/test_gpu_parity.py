@@ -42,7 +42,8 @@ def test_copy_all_variants(trove, cuda, s):
     src = rand_bytes(n * s, s)
     want = oracle.copy(src, n, s)
     d_src = dev(src)
-    cases = (("shfl", TILES), ("direct", (64,))) + ((("lds", (64,)),) if T.lds_supported(s) else ())
+    cases = (("shfl", TILES), ("direct", (64,)), ("direct_launch", (64,))) + \
+        ((("lds", (64,)),) if T.lds_supported(s) else ())
     for variant, tiles in cases:
         for tile in tiles:
             d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
@@ -68,13 +69,13 @@ def test_seeded_random_cases(trove, cuda):
         lds = ("lds",) if T.lds_supported(s) else ()
         tile = 64
         if op == "copy":
-            variant = str(rng.choice(("shfl", "direct", "auto") + lds))
+            variant = str(rng.choice(("shfl", "direct", "direct_launch", "auto") + lds))
             if variant == "shfl":
                 tile = int(rng.choice(TILES))
             elif variant == "auto":
                 tile = int(rng.choice((64, 32)))
         else:
-            variant = str(rng.choice(("shfl", "tile", "direct", "auto") + lds))
+            variant = str(rng.choice(("shfl", "tile", "direct", "direct_launch", "auto") + lds))
         what = (case, op, s, n, variant, tile)
         src = rand_bytes(n * s, 5000 + case)
         if op == "copy":
@@ -131,7 +132,7 @@ def test_gather_all_variants(trove, cuda, s):
     for name, idx in _patterns(n, n_src, s).items():
         want = oracle.gather(src, idx, s)
         d_idx = dev(idx)
-        for variant in ("shfl", "tile", "direct") + (("lds",) if T.lds_supported(s) else ()):
+        for variant in ("shfl", "tile", "direct", "direct_launch") + (("lds",) if T.lds_supported(s) else ()):
             d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
             T.gather(d_src, d_idx, d_dst, n, s, variant=variant)
             assert np.array_equal(host(d_dst), want), (s, name, variant)
@@ -146,7 +147,7 @@ def test_scatter_all_variants(trove, cuda, s):
     for name, idx in (("perm", np.random.default_rng(s).permutation(n).astype(np.int32)),
                       ("identity", np.arange(n, dtype=np.int32))):
         want = oracle.scatter(src, idx, init, s)
-        for variant in ("shfl", "tile", "direct") + (("lds",) if T.lds_supported(s) else ()):
+        for variant in ("shfl", "tile", "direct", "direct_launch") + (("lds",) if T.lds_supported(s) else ()):
             d_dst = dev(init.copy())
             T.scatter(d_src, dev(idx), d_dst, n, s, variant=variant)
             assert np.array_equal(host(d_dst), want), (s, name, variant)
@@ -156,7 +157,7 @@ def test_scatter_all_variants(trove, cuda, s):
 def test_gather_scatter_sub_wavefront_tiles(trove, cuda, s):
     # trove_gather / trove_scatter with tile 32..2: coalesced_ptr on tiles of
     # that many lanes (AUTO resolves to it), a partial last group included;
-    # DIRECT ignores the width.
+    # DIRECT and DIRECT_LAUNCH ignore the width.
     n = 64 * 29 + 37
     rng = np.random.default_rng(900 + s)
     src = rand_bytes(n * s, 910 + s)
@@ -165,7 +166,7 @@ def test_gather_scatter_sub_wavefront_tiles(trove, cuda, s):
     want_s = oracle.scatter(src, perm, np.zeros(n * s, dtype=np.uint8), s)
     d_src, d_perm = dev(src), dev(perm)
     for tile in (32, 16, 8, 4, 2):
-        for variant in ("shfl", "auto", "direct"):
+        for variant in ("shfl", "auto", "direct", "direct_launch"):
             d_dst = torch.zeros(n * s, dtype=torch.uint8, device=cuda)
             T.gather(d_src, d_perm, d_dst, n, s, variant=variant, tile=tile)
             assert np.array_equal(host(d_dst), want_g), ("gather", s, tile, variant)
@@ -207,7 +208,7 @@ def test_gather_scatter_fewer_than_a_group(trove, cuda, s):
     n_src = 200
     src = rand_bytes(n_src * s, 60 + s)
     d_src = dev(src)
-    variants = ("shfl", "tile", "direct") + (("lds",) if T.lds_supported(s) else ())
+    variants = ("shfl", "tile", "direct", "direct_launch") + (("lds",) if T.lds_supported(s) else ())
     for n in (1, 5, 63, 64, 65, 129):
         idx = np.random.default_rng(n + s).integers(0, n_src, size=n, dtype=np.int32)
         perm = np.random.default_rng(n + s + 1).permutation(n).astype(np.int32)

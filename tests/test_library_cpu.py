@@ -109,7 +109,10 @@ def test_validation_before_launch():
     assert L.trove_gather(fake, fake + 2, fake, 10, 20, 3, 0, None) == T.ERR_ALIGN
     assert L.trove_gather(fake, 0, fake, 10, 20, 3, 0, None) == T.ERR_INVALID
     assert L.trove_scatter(fake, fake, fake, 10, 20, 9, 0, None) == T.ERR_VARIANT
-    assert L.trove_gather(fake, fake, fake, 10, 20, T.VARIANT_LDS + 1, 0, None) == T.ERR_VARIANT
+    assert L.trove_gather(fake, fake, fake, 10, 20, T.VARIANT_DIRECT_LAUNCH + 1, 0, None) == T.ERR_VARIANT
+    # The launch-equal naive variant is for copy, gather and scatter only.
+    assert L.trove_store_counting(fake, 10, 20, T.VARIANT_DIRECT_LAUNCH, 0, None) == T.ERR_VARIANT
+    assert L.trove_load_sum(fake, fake, 10, 20, T.VARIANT_DIRECT_LAUNCH, 0, None) == T.ERR_VARIANT
     assert L.trove_gather(0, 0, 0, 0, 20, 3, 0, None) == T.OK
     assert L.trove_store_counting(fake, 10, 6, 1, 0, None) == T.ERR_SIZE
     assert L.trove_load_sum(fake, fake + 1, 10, 20, 1, 0, None) == T.ERR_ALIGN

@@ -39,8 +39,9 @@ VARIANT_DIRECT = 2
 VARIANT_TILE = 3
 
 VARIANT_LDS = 4
+VARIANT_DIRECT_LAUNCH = 5
 VARIANTS = {"auto": VARIANT_AUTO, "shfl": VARIANT_SHFL, "direct": VARIANT_DIRECT, "tile": VARIANT_TILE,
-            "lds": VARIANT_LDS}
+            "lds": VARIANT_LDS, "direct_launch": VARIANT_DIRECT_LAUNCH}
 # The LDS-staged variant takes struct sizes that are multiples of 4 bytes up
 # to this many bytes (its stage is 64 structs per wavefront).
 LDS_MAX_STRUCT_BYTES = 512

@@ -109,7 +109,7 @@ static const size_entry* find_size(int bytes) {
     return nullptr;
 }
 
-static bool valid_variant(int v) { return v >= TROVE_VARIANT_AUTO && v <= TROVE_VARIANT_LDS; }
+static bool valid_variant(int v) { return v >= TROVE_VARIANT_AUTO && v <= TROVE_VARIANT_DIRECT_LAUNCH; }
 // Tile widths of the warp-contiguous API: 0 (= 64) or a power of two 2..64.
 static bool valid_tile(int tile) { return tile == 0 || (tile >= 2 && tile <= 64 && (tile & (tile - 1)) == 0); }
 
@@ -240,7 +240,8 @@ int trove_store_counting(void* dst, int64_t n, int struct_bytes, int variant, in
     const size_entry* e = find_size(struct_bytes);
     if (!e || struct_bytes % 4 != 0) return TROVE_ERR_SIZE;
     if (n < 0) return TROVE_ERR_INVALID;
-    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS || !valid_tile(tile))
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS ||
+        variant == TROVE_VARIANT_DIRECT_LAUNCH || !valid_tile(tile))
         return TROVE_ERR_VARIANT;
     if (n == 0) return TROVE_OK;
     if (!dst) return TROVE_ERR_INVALID;
@@ -253,7 +254,8 @@ int trove_load_sum(const void* src, uint32_t* out, int64_t n, int struct_bytes, 
     const size_entry* e = find_size(struct_bytes);
     if (!e || struct_bytes % 4 != 0) return TROVE_ERR_SIZE;
     if (n < 0) return TROVE_ERR_INVALID;
-    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS || !valid_tile(tile))
+    if (!valid_variant(variant) || variant == TROVE_VARIANT_TILE || variant == TROVE_VARIANT_LDS ||
+        variant == TROVE_VARIANT_DIRECT_LAUNCH || !valid_tile(tile))
         return TROVE_ERR_VARIANT;
     if (n == 0) return TROVE_OK;
     if (!src || !out) return TROVE_ERR_INVALID;

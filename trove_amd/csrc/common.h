@@ -55,6 +55,16 @@ constexpr int auto_scatter_variant(int b) {
 constexpr bool auto_scatter_by_inverse(int b) { return b % 4 == 0 && b >= 12 && b <= 60; }
 constexpr int kWavesPerBlock = kBlock / 64;
 
+// Cache policy P for a per-lane access (each lane moving its own struct word
+// by word): a struct of several words per lane touches each line from
+// several instructions, which only the cache turns into whole-line traffic
+// (nontemporal per-lane words re-fetch or partially write the line for each
+// word: 1.6 TB/s against 4.5 for ordinary accesses at 60-byte copies on
+// MI355X), so P applies to one-word structs only.
+template <typename T, typename P>
+using per_lane_policy =
+    typename std::conditional<trove::detail::word_layout<T>::m == 1, P, trove::cached>::type;
+
 // The struct type moved for a given byte size: k 32-bit words when the size
 // is a multiple of 4 (the reference benchmark's trove::array<int, k>),
 // otherwise raw bytes.

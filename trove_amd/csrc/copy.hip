@@ -5,6 +5,11 @@
 //  LDS    - the LDS-staged transposition (load/store_warp_contiguous_lds):
 //           dwordx4 tile traffic through a per-wavefront LDS stage.
 //  DIRECT - the naive per-lane struct copy the compiler generates.
+//  DIRECT_LAUNCH - the naive per-lane struct access (one struct per lane,
+//           its words loaded and stored by the lane) in SHFL's launch: the
+//           same grid, XCD order and residency cap, and SHFL's streaming
+//           policy where a per-lane access allows it (one-word structs), so
+//           that it differs from SHFL only by the transposition.
 // Also the reference benchmark's store-only (counting structs) and load-only
 // (per-struct word sum) kernels, tests/benchmark.cu:52-88.
 //
@@ -30,7 +35,27 @@ struct copy_shape {
     static constexpr bool xcd_map = m > 1 && sizeof(T) >= 20;
 };
 
-template <typename T, int TILE, int UNROLL, int B, bool XCD>
+// One 64-struct group at p (the group's first struct): through the
+// warp-contiguous pair (NAIVE false) or per lane (NAIVE true; streaming for
+// one-word structs only, see per_lane_policy).
+template <typename T, int TILE, bool NAIVE>
+__device__ __forceinline__ T copy_group_load(const T* p, int lane) {
+    if constexpr (NAIVE) {
+        return trove::detail::divergent_load<per_lane_policy<T, trove::streaming>>(p + lane);
+    } else {
+        return trove::load_warp_contiguous<TILE, trove::streaming>(p + lane);
+    }
+}
+template <typename T, int TILE, bool NAIVE>
+__device__ __forceinline__ void copy_group_store(const T& v, T* p, int lane) {
+    if constexpr (NAIVE) {
+        trove::detail::divergent_store<per_lane_policy<T, trove::streaming>>(v, p + lane);
+    } else {
+        trove::store_warp_contiguous<TILE, trove::streaming>(v, p + lane);
+    }
+}
+
+template <typename T, int TILE, int UNROLL, int B, bool XCD, bool NAIVE = false>
 __global__ __launch_bounds__(B) void k_copy_shfl(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
     const int lane = trove::lane_id();
     const int64_t groups = n >> 6;
@@ -44,8 +69,8 @@ __global__ __launch_bounds__(B) void k_copy_shfl(const T* __restrict__ src, T* _
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
                 if (g0 + u < groups) {
-                    const T v = trove::load_warp_contiguous<TILE, trove::streaming>(src + ((g0 + u) << 6) + lane);
-                    trove::store_warp_contiguous<TILE, trove::streaming>(v, dst + ((g0 + u) << 6) + lane);
+                    const T v = copy_group_load<T, TILE, NAIVE>(src + ((g0 + u) << 6), lane);
+                    copy_group_store<T, TILE, NAIVE>(v, dst + ((g0 + u) << 6), lane);
                 }
             }
         }
@@ -56,15 +81,13 @@ __global__ __launch_bounds__(B) void k_copy_shfl(const T* __restrict__ src, T* _
     for (; g + UNROLL <= groups; g += step) {
         T v[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u)
-            v[u] = trove::load_warp_contiguous<TILE, trove::streaming>(src + ((g + u) << 6) + lane);
+        for (int u = 0; u < UNROLL; ++u) v[u] = copy_group_load<T, TILE, NAIVE>(src + ((g + u) << 6), lane);
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u)
-            trove::store_warp_contiguous<TILE, trove::streaming>(v[u], dst + ((g + u) << 6) + lane);
+        for (int u = 0; u < UNROLL; ++u) copy_group_store<T, TILE, NAIVE>(v[u], dst + ((g + u) << 6), lane);
     }
     for (; g < groups; ++g) {
-        const T v = trove::load_warp_contiguous<TILE, trove::streaming>(src + (g << 6) + lane);
-        trove::store_warp_contiguous<TILE, trove::streaming>(v, dst + (g << 6) + lane);
+        const T v = copy_group_load<T, TILE, NAIVE>(src + (g << 6), lane);
+        copy_group_store<T, TILE, NAIVE>(v, dst + (g << 6), lane);
     }
 }
 
@@ -119,10 +142,17 @@ __global__ __launch_bounds__(64) void k_copy_lds(const T* __restrict__ src, T* _
 // Partial trailing group (n % 64 structs), one wavefront: the active lanes go
 // through the tile cascade. A kernel of its own, so that the streaming kernel
 // does not carry the cascade's registers (up to 512 per lane at 1 KiB).
-template <typename T>
+template <typename T, bool NAIVE = false>
 __global__ __launch_bounds__(64) void k_copy_tail(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
     const int64_t i = ((n >> 6) << 6) + trove::lane_id();
-    if (i < n) trove::store(trove::load(src + i), dst + i);
+    if (i < n) {
+        if constexpr (NAIVE) {
+            typedef per_lane_policy<T, trove::streaming> P;
+            trove::detail::divergent_store<P>(trove::detail::divergent_load<P>(src + i), dst + i);
+        } else {
+            trove::store(trove::load(src + i), dst + i);
+        }
+    }
 }
 
 template <typename T>
@@ -131,20 +161,20 @@ __global__ __launch_bounds__(kBlock) void k_copy_direct(const T* __restrict__ sr
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
-template <typename T, int TILE>
+template <typename T, int TILE, bool NAIVE = false>
 static int launch_copy_shfl(const T* src, T* dst, int64_t n, hipStream_t s) {
     typedef copy_shape<T> C;
     const int64_t units = ((n >> 6) + C::unroll - 1) / C::unroll;
     const int64_t blocks = grid_for(units, C::block / 64);
     const size_t lds = stream_lds_reservation((int64_t)C::unroll * 64 * (int64_t)sizeof(T) * (C::block / 64));
     if (n >= 64) {
-        hipLaunchKernelGGL((k_copy_shfl<T, TILE, C::unroll, C::block, C::xcd_map>), dim3((unsigned)blocks),
+        hipLaunchKernelGGL((k_copy_shfl<T, TILE, C::unroll, C::block, C::xcd_map, NAIVE>), dim3((unsigned)blocks),
                            dim3(C::block), lds, s, src, dst, n);
         const int rc = launch_status();
         if (rc != TROVE_OK) return rc;
     }
     if (n & 63) {
-        hipLaunchKernelGGL((k_copy_tail<T>), dim3(1), dim3(64), 0, s, src, dst, n);
+        hipLaunchKernelGGL((k_copy_tail<T, NAIVE>), dim3(1), dim3(64), 0, s, src, dst, n);
         return launch_status();
     }
     return TROVE_OK;
@@ -205,6 +235,7 @@ int copy_impl(const void* src_, void* dst_, int64_t n, int variant, int tile, hi
         hipLaunchKernelGGL((k_copy_direct<T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, src, dst, n);
         return launch_status();
     }
+    if (variant == TROVE_VARIANT_DIRECT_LAUNCH) return launch_copy_shfl<T, 64, true>(src, dst, n, s);
     return TROVE_ERR_VARIANT;
 }
 

@@ -152,6 +152,61 @@ __global__ __launch_bounds__(kBlock) void k_gather_direct(const T* __restrict__ 
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[idx[i]];
 }
 
+// The naive gather and scatter in the tile path's launch (same grid, work-
+// group size and groups in flight; the tile path's cache policies where a
+// per-lane access allows them), each lane moving its own struct word by
+// word: what the tile path adds over a per-lane access on equal launch terms.
+template <typename T, int UNROLL, int B>
+__global__ __launch_bounds__(B) void k_gather_direct_launch(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                                                            T* __restrict__ dst, int64_t n) {
+    namespace d = trove::detail;
+    typedef trove::streaming S;
+    typedef per_lane_policy<T, row_read_policy<T>> RP;
+    typedef per_lane_policy<T, S> WP;
+    const int lane = trove::lane_id();
+    const int64_t groups = (n + 63) >> 6;
+    for (int64_t g0 = wave_index<B>() * UNROLL; g0 < groups; g0 += wave_count<B>() * UNROLL) {
+        if (((g0 + UNROLL) << 6) <= n) {
+            int32_t ix[UNROLL];
+            T v[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) ix[u] = d::load_word<S>(idx + ((g0 + u) << 6) + lane);
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) v[u] = d::divergent_load<RP>(src + ix[u]);
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) d::divergent_store<WP>(v[u], dst + ((g0 + u) << 6) + lane);
+        } else {
+            for (int64_t i = (g0 << 6) + lane; i < n; i += 64)
+                d::divergent_store<WP>(d::divergent_load<RP>(src + idx[i]), dst + i);
+        }
+    }
+}
+
+template <typename T, int UNROLL>
+__global__ __launch_bounds__(kBlock) void k_scatter_direct_launch(const T* __restrict__ src,
+                                                                  const int32_t* __restrict__ idx,
+                                                                  T* __restrict__ dst, int64_t n) {
+    namespace d = trove::detail;
+    typedef trove::streaming S;
+    const int lane = trove::lane_id();
+    const int64_t groups = (n + 63) >> 6;
+    for (int64_t g0 = wave_index() * UNROLL; g0 < groups; g0 += wave_count() * UNROLL) {
+        if (((g0 + UNROLL) << 6) <= n) {
+            int32_t ix[UNROLL];
+            T v[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                ix[u] = d::load_word<S>(idx + ((g0 + u) << 6) + lane);
+                v[u] = d::divergent_load<per_lane_policy<T, S>>(src + ((g0 + u) << 6) + lane);
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) d::divergent_store(v[u], dst + ix[u]);
+        } else {
+            for (int64_t i = (g0 << 6) + lane; i < n; i += 64) d::divergent_store(d::divergent_load(src + i), dst + idx[i]);
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scatter_shfl(const T* __restrict__ src, const int32_t* __restrict__ idx,
                                                          T* __restrict__ dst, int64_t n) {
@@ -242,7 +297,7 @@ int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int
     if (tile != 0 && tile != 64) {  // a sub-wavefront tile asks for coalesced_ptr at that width
         if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL)
             return launch_shfl_tile<T, false>(src, idx, dst, n, tile, s);
-        if (variant != TROVE_VARIANT_DIRECT) return TROVE_ERR_VARIANT;
+        if (variant != TROVE_VARIANT_DIRECT && variant != TROVE_VARIANT_DIRECT_LAUNCH) return TROVE_ERR_VARIANT;
     }
     if (variant == TROVE_VARIANT_AUTO) variant = auto_gather_variant(B);
     if (variant == TROVE_VARIANT_TILE && !tiled) variant = TROVE_VARIANT_DIRECT;
@@ -278,6 +333,12 @@ int gather_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, int
                            idx, dst, n);
         return launch_status();
     }
+    if (variant == TROVE_VARIANT_DIRECT_LAUNCH) {
+        hipLaunchKernelGGL((k_gather_direct_launch<T, G::unroll, G::block>),
+                           dim3((unsigned)grid_for(units > 0 ? units : 1, G::block / 64)), dim3(G::block), 0, s, src,
+                           idx, dst, n);
+        return launch_status();
+    }
     return TROVE_ERR_VARIANT;
 }
 
@@ -290,7 +351,7 @@ int scatter_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, in
     if (tile != 0 && tile != 64) {  // a sub-wavefront tile asks for coalesced_ptr at that width
         if (variant == TROVE_VARIANT_AUTO || variant == TROVE_VARIANT_SHFL)
             return launch_shfl_tile<T, true>(src, idx, dst, n, tile, s);
-        if (variant != TROVE_VARIANT_DIRECT) return TROVE_ERR_VARIANT;
+        if (variant != TROVE_VARIANT_DIRECT && variant != TROVE_VARIANT_DIRECT_LAUNCH) return TROVE_ERR_VARIANT;
     }
     // 4-byte rows: one group per wavefront. With four, the streaming loads
     // of the next groups slow the random 4-byte stores to half the rate
@@ -328,6 +389,12 @@ int scatter_impl(const void* src_, const int32_t* idx, void* dst_, int64_t n, in
     if (variant == TROVE_VARIANT_DIRECT) {
         hipLaunchKernelGGL((k_scatter_direct<T>), dim3((unsigned)grid_for((n + 63) / 64)), dim3(kBlock), 0, s, src,
                            idx, dst, n);
+        return launch_status();
+    }
+    if (variant == TROVE_VARIANT_DIRECT_LAUNCH) {
+        const int64_t all_units = (((n + 63) >> 6) + U - 1) / U;
+        hipLaunchKernelGGL((k_scatter_direct_launch<T, U>), dim3((unsigned)grid_for(all_units > 0 ? all_units : 1)),
+                           dim3(kBlock), 0, s, src, idx, dst, n);
         return launch_status();
     }
     return TROVE_ERR_VARIANT;
