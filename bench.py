@@ -516,19 +516,20 @@ def traffic_from(fetch_dir: str, write_dir: str, legs_json: str, out_path: str =
                                 capture_output=True, text=True, timeout=20).stdout.strip() or None
     except (OSError, subprocess.SubprocessError):
         commit = None
-    kernels, table = {}, []
+    kernels, reads, writes, table = {}, {}, {}, []
     for name in sorted(per):
         n_, fk, wk, nbytes = per[name]
         fk, wk = fk / n_, wk / n_
         rd, wr = fk * 1024 * f_fac, wk * 1024 * w_fac
         kernels[name] = int(round(rd + wr))
+        reads[name], writes[name] = int(round(rd)), int(round(wr))
         table.append(f"| `{name}` | {fk:.0f} | {wk:.0f} | {rd / 1e9:.3f} | {wr / 1e9:.3f} | {nbytes / 1e9:.3f} | "
                      f"{(rd + wr) / nbytes:.2f} |")
     out = {"unit": "HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated on the plain copy)",
            "kernel_sources_sha256": kernel_sources_sha256(), "kernels_commit": commit,
            "calibration": {"fetch_factor": round(f_fac, 4), "write_factor": round(w_fac, 4),
                            "bytes_each_way": int(each_way)},
-           "kernels": kernels}
+           "kernels": kernels, "read": reads, "written": writes}
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")
@@ -541,16 +542,30 @@ def traffic_from(fetch_dir: str, write_dir: str, legs_json: str, out_path: str =
     return 0
 
 
-def load_traffic(path: str = TRAFFIC_PATH):
+def load_traffic(path: str = TRAFFIC_PATH, split: bool = False):
     """(kernels, stale): the committed per-leg HBM bytes, and whether they were
-    measured on other kernel sources than these (then kernels is None)."""
+    measured on other kernel sources than these (then kernels is None). With
+    split, kernels maps each leg to (read bytes, written bytes)."""
     if not os.path.exists(path):
         return None, True
     with open(path) as f:
         tr = json.load(f)
     if tr.get("kernel_sources_sha256") != kernel_sources_sha256():
         return None, True
+    if split:
+        rd, wr = tr.get("read", {}), tr.get("written", {})
+        return {k: (rd[k], wr[k]) for k in rd if k in wr}, False
     return tr.get("kernels", {}), False
+
+
+def hbm_bound_ms(read_bytes, written_bytes, index_bytes, random_rows, random_gbs, stream_gbs):
+    """Time the HBM needs for a launch's measured traffic: the random row
+    fills (the read bytes beyond the index stream, when random_rows) at the
+    random-line rate, everything else (index stream, contiguous reads, the
+    writes) at the streaming-copy rate."""
+    random = max(read_bytes - index_bytes, 0) if random_rows else 0
+    streaming = read_bytes + written_bytes - random
+    return (random / (random_gbs * 1e9) + streaming / (stream_gbs * 1e9)) * 1e3
 
 
 def main():
@@ -826,13 +841,21 @@ def main():
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes (profiles/traffic.json; null when those
     # were measured on other kernel sources than these).
-    traffic_table, traffic_stale = load_traffic()
-    traffic = traffic_table.get(dname) if traffic_table else None
+    traffic_table, traffic_stale = load_traffic(split=True)
+    rw = traffic_table.get(dname) if traffic_table else None
+    traffic = rw[0] + rw[1] if rw else None
     hbm_gbs = traffic / (dms / dl * 1e-3) / 1e9 if traffic else None
-    # The measured ceiling matching the dominant kernel: random line fills
-    # for a gather, the plain streaming copy for a copy.
-    ceiling_name = "random_lines_ceiling" if dname.startswith("gather") else "memcpy_ceiling"
-    ceiling_gbs = naive.get(ceiling_name, {}).get("gbs")
+    # The measured ceilings matching the dominant kernel's traffic: a
+    # gather's random row fills at the random-line read rate, its index
+    # stream and output writes (a copy's whole traffic) at the streaming-copy
+    # rate; ceiling_frac = that bound's time / the kernel's time.
+    random_gbs = naive.get("random_lines_ceiling", {}).get("gbs")
+    stream_gbs = naive.get("memcpy_ceiling", {}).get("gbs")
+    is_gather = dname.startswith("gather")
+    bound_ms = None
+    if rw and stream_gbs and (random_gbs or not is_gather):
+        bound_ms = hbm_bound_ms(rw[0], rw[1], 4 * n if is_gather else 0, is_gather, random_gbs or 1.0, stream_gbs)
+    ceiling_gbs = round(traffic / (bound_ms * 1e-3) / 1e9, 1) if bound_ms else None
     # Random-row write ceilings and the scatters of those row sizes against
     # them (row bytes written per second by the tile scatter / by the ceiling
     # kernel's better store policy).
@@ -946,9 +969,11 @@ def main():
             "traffic": traffic,
             "traffic_stale": traffic_stale,
             "hbm_gbs": round(hbm_gbs, 1) if hbm_gbs else None,
-            "ceiling": ceiling_name,
+            "ceiling": ("random row fills at random_line_ceiling_gbs + index and output streams at "
+                        "memcpy_ceiling_gbs" if is_gather else "traffic at memcpy_ceiling_gbs"),
+            "ceiling_ms": round(bound_ms, 4) if bound_ms else None,
             "ceiling_gbs": ceiling_gbs,
-            "ceiling_frac": round(hbm_gbs / ceiling_gbs, 3) if hbm_gbs and ceiling_gbs else None,
+            "ceiling_frac": round(bound_ms / (dms / dl), 3) if bound_ms else None,
         },
         "cpu_baseline": cpu,
         "memcpy_ceiling_gbs": naive.get("memcpy_ceiling", {}).get("gbs"),

@@ -80,6 +80,12 @@ def test_traffic_from_attributes_dispatches_to_legs(tmp_path):
     assert tr["kernel_sources_sha256"] == bench.kernel_sources_sha256()
     kernels, stale = bench.load_traffic(str(out))
     assert not stale and kernels == k
+    split, _ = bench.load_traffic(str(out), split=True)
+    assert split["gather_b"] == (3 * 1024 * 2, 6 * 1024) and split["memcpy_ceiling"] == (4096, 4096)
+    # The HBM bound prices random row fills (reads beyond the index stream) at
+    # the random-line rate and the rest at the streaming rate.
+    assert bench.hbm_bound_ms(10e9, 4e9, 1e9, True, 5.0, 6.0) == pytest.approx((9 / 5 + 5 / 6) * 1e3)
+    assert bench.hbm_bound_ms(4e9, 4e9, 0, False, 5.0, 8.0) == pytest.approx(1e3)
     tr["kernel_sources_sha256"] = "0" * 64  # measured on other sources: not used
     json.dump(tr, open(out, "w"))
     assert bench.load_traffic(str(out)) == (None, True)
