@@ -99,14 +99,16 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB_PATH
 
 
-DEVTESTS_SRC = os.path.join(ROOT, "tests", "host", "devtests.hip")
+DEVTESTS_SRC = [os.path.join(ROOT, "tests", "host", f) for f in ("devtests.hip", "queue_primitives.hip")]
 DEVTESTS_LIB = os.path.join(ROOT, "tests", "host", "libtrove_devtests.so")
 
 
 def build_devtests() -> str:
-    """Compiles the device-API unit-test kernels (tests/host/devtests.hip)."""
+    """Compiles the device-API unit-test kernels (tests/host/devtests.hip)
+    and the queue-primitive measurements (tests/host/queue_primitives.hip)
+    into one test library."""
     tmp = DEVTESTS_LIB + ".tmp"
-    rc, cmd, out = _compile([HIPCC, *HIP_FLAGS, "-shared", DEVTESTS_SRC, "-o", tmp])
+    rc, cmd, out = _compile([HIPCC, *HIP_FLAGS, "-shared", *DEVTESTS_SRC, "-o", tmp])
     if rc != 0:
         print(cmd)
         print(out)
