@@ -49,8 +49,9 @@ def parse_args():
                    help="structs per size in the CPU sample (log2; 26 = the sweep's N, about 10-15 s of CPU work)")
     p.add_argument("--no-naive", action="store_true", help="skip the naive-kernel comparison")
     p.add_argument("--seed", type=int, default=20260101)
-    p.add_argument("--config", choices=("all", "sweep", "batched", "served"), default="all",
-                   help="all: the sweep line with batched and served summaries attached")
+    p.add_argument("--config", choices=("all", "sweep", "batched", "served", "line-order"), default="all",
+                   help="all: the sweep line with batched and served summaries attached; line-order: the "
+                        "random-line read rate with the line indices sorted within windows")
     p.add_argument("--jobs", type=int, default=1024, help="batched: number of gather jobs")
     p.add_argument("--tune-table", type=str, default=None, help="batched: tuner table to load instead of tuning")
     p.add_argument("--serve-slots", type=int, default=1024, help="served: request slots of the server")
@@ -568,6 +569,40 @@ def hbm_bound_ms(read_bytes, written_bytes, index_bytes, random_rows, random_gbs
     return (random / (random_gbs * 1e9) + streaming / (stream_gbs * 1e9)) * 1e3
 
 
+
+def bench_line_order(args, torch, T):
+    """2^26 random 128-byte line reads from a 4 GiB source (the random-line
+    ceiling kernel), with the same line indices in random order and sorted
+    within consecutive windows of 64 to 2^20 indices, then fully sorted:
+    whether an order a gather could stage (DRAM page locality, address
+    translation) raises the rate random row fills are served at."""
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    n_lines = (4 << 30) // 128
+    n_reads = 1 << 26
+    src = torch.empty(n_lines * 32, dtype=torch.int32, device=dev)
+    T.fill_words_device(src, src.numel(), args.seed + 40)
+    out = torch.empty(n_reads, dtype=torch.int32, device=dev)
+    base = T.gen_indices(n_reads, n_lines, "random", args.seed + 41)
+    rates = {}
+    for w in ("random", 64, 1 << 10, 1 << 14, 1 << 18, 1 << 20, "sorted"):
+        if w == "random":
+            order = base
+        elif w == "sorted":
+            order = np.sort(base)
+        else:
+            order = np.sort(base.reshape(-1, w), axis=1).reshape(-1)
+        d_idx = torch.from_numpy(np.ascontiguousarray(order)).to(dev)
+        T.random_lines_ceiling(src, n_lines, d_idx, out, n_reads)
+        ms = statistics.median(timed_runs(torch, lambda: T.random_lines_ceiling(src, n_lines, d_idx, out, n_reads), 5))
+        rates[str(w)] = round(128 * n_reads / (ms * 1e-3) / 1e9, 1)
+        log(f"bench line-order: window {w}: {rates[str(w)]} GB/s")
+        del d_idx
+    return {"workload": "2^26 reads of random 128-byte lines of a 4 GiB source; indices in random order, "
+                        "sorted within windows of the given length, or fully sorted; median of 5",
+            "gbs_by_window": rates}
+
 def main():
     args = parse_args()
     if args.traffic_from:
@@ -596,6 +631,10 @@ def main():
         return 0
     if args.config == "served":
         print(json.dumps({"metric": "served gather p99 latency", "served": bench_served(args, torch, T)}),
+              flush=True)
+        return 0
+    if args.config == "line-order":
+        print(json.dumps({"metric": "random 128-byte line reads GB/s", "line_order": bench_line_order(args, torch, T)}),
               flush=True)
         return 0
     dev = torch.device("cuda:0")
