@@ -110,9 +110,11 @@ int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int 
 int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant, int tile,
                   void* stream);
 
-/* inv[perm[i]] = i, i < n, for a permutation perm of [0, n) (not checked).
- * No reference counterpart: with it, a scatter through one permutation can
- * be run as gathers through the inverse (see trove_scatter_permutation). */
+/* inv[perm[i]] = i, i < n, for a permutation perm of [0, n) (not checked;
+ * an entry outside [0, n) is skipped, and entries of inv that no perm[i]
+ * names keep their contents). No reference counterpart: with it, a scatter
+ * through one permutation can be run as gathers through the inverse (see
+ * trove_scatter_permutation). */
 int trove_invert_permutation(const int32_t* perm, int32_t* inv, int64_t n, void* stream);
 
 /* dst[perm[i]] = src[i], i < n, for a permutation perm of [0, n) (dst holds
@@ -123,8 +125,10 @@ int trove_invert_permutation(const int32_t* perm, int32_t* inv, int64_t n, void*
  * to 60 bytes (multiples of 4): random row writes that cover part of a
  * 64-byte memory granule are several times slower than random row reads.
  * Other sizes, or workspace == NULL, run trove_scatter(AUTO). The
- * reference's random scatter benchmark scatters through a permutation
- * (tests/benchmark.cu:101-110, 244-258). */
+ * workspace is zeroed before the inversion, so that for a perm that is not a
+ * permutation the result is unspecified but every access stays inside src,
+ * dst and workspace. The reference's random scatter benchmark scatters
+ * through a permutation (tests/benchmark.cu:101-110, 244-258). */
 int trove_scatter_permutation(const void* src, const int32_t* perm, void* dst, int64_t n, int struct_bytes,
                               int32_t* workspace, void* stream);
 /* 1 when trove_scatter_permutation with a workspace inverts and gathers at

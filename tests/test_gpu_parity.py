@@ -183,6 +183,55 @@ def test_invert_permutation(trove, cuda):
         assert np.array_equal(host(inv), np.argsort(perm).astype(np.int32)), n
 
 
+def test_invert_skips_entries_outside_the_range(trove, cuda):
+    # Entries outside [0, n) are skipped and unnamed inverse entries keep
+    # their contents (the kernel's own bounds check; nothing is written
+    # outside inv).
+    n = 64 * 50 + 9
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(n).astype(np.int32)
+    bad = rng.choice(n, size=200, replace=False)
+    perm[bad[:100]] = n + rng.integers(0, 1 << 20, size=100)
+    perm[bad[100:]] = -1 - rng.integers(0, 1 << 20, size=100)
+    inv = torch.full((n,), -7, dtype=torch.int32, device=cuda)
+    T.invert_permutation(dev(perm), inv, n)
+    want = np.full(n, -7, dtype=np.int32)
+    ok = (perm >= 0) & (perm < n)
+    want[perm[ok]] = np.nonzero(ok)[0]
+    assert np.array_equal(host(inv), want)
+
+
+@pytest.mark.parametrize("s", [12, 60])
+def test_scatter_permutation_of_a_non_permutation_stays_in_bounds(trove, cuda, s):
+    # Not a permutation (duplicates and entries outside [0, n)): the result is
+    # unspecified, but the workspace is zeroed before the inversion, so a row
+    # nothing maps to gathers row 0 and a row one in-range entry maps to
+    # gathers that entry's source row.
+    k = s // 4
+    n = 64 * 40 + 13
+    rng = np.random.default_rng(s)
+    perm = rng.permutation(n).astype(np.int32)
+    perm[rng.choice(n, size=150, replace=False)] = perm[rng.choice(n, size=150, replace=False)]
+    perm[rng.choice(n, size=50, replace=False)] = n + 3
+    words = T.gen_words(n * k, 17 + s)
+    src = dev(words.view(np.int32))
+    dst = torch.full((n * k,), -1, dtype=torch.int32, device=cuda)
+    ws = torch.full((n,), 12345, dtype=torch.int32, device=cuda)
+    assert T.scatter_permutation_inverts(s)
+    T.scatter_permutation(src, dev(perm), dst, n, s, workspace=ws)
+    got = host(dst).reshape(n, k)
+    rows = words.view(np.int32).reshape(n, k)
+    ok = (perm >= 0) & (perm < n)
+    counts = np.bincount(perm[ok], minlength=n)
+    single = np.nonzero(counts == 1)[0]
+    first = np.full(n, -1, dtype=np.int64)
+    first[perm[ok][::-1]] = np.nonzero(ok)[0][::-1]
+    assert np.array_equal(got[single], rows[first[single]])
+    assert np.array_equal(got[counts == 0], np.broadcast_to(rows[0], (int((counts == 0).sum()), k)))
+    for j in np.nonzero(counts > 1)[0]:
+        assert any(np.array_equal(got[j], rows[i]) for i in np.nonzero(perm == j)[0]), j
+
+
 @pytest.mark.parametrize("s", SIZES)
 def test_scatter_permutation_all_sizes(trove, cuda, s):
     # dst[perm[i]] = src[i]: the oracle's scatter, through the inversion and

@@ -278,16 +278,31 @@ int random_rows_write_launch(void* buf, int64_t n_rows, int row_bytes, const int
 // inv[perm[i]] = i for a permutation perm of [0, n): one 4-byte random write
 // per element, as ordinary (cached) stores. The naive 4-byte scatter, the
 // same random writes, moves 2^26 elements in 0.86 ms on MI355X (78 G
-// writes/s), several times the rate of random 32-byte rows.
+// writes/s), several times the rate of random 32-byte rows. An entry outside
+// [0, n) is skipped, so that no input writes outside inv.
 __global__ __launch_bounds__(kBlock) void k_invert_permutation(const int32_t* __restrict__ perm,
                                                                int32_t* __restrict__ inv, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-        inv[__builtin_nontemporal_load(perm + i)] = (int32_t)i;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const int32_t p = __builtin_nontemporal_load(perm + i);
+        if (p >= 0 && (int64_t)p < n) inv[p] = (int32_t)i;
+    }
 }
 
-int invert_permutation_launch(const int32_t* perm, int32_t* inv, int64_t n, hipStream_t s) {
+__global__ __launch_bounds__(kBlock) void k_zero_words(int32_t* __restrict__ out, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) out[i] = 0;
+}
+
+int invert_permutation_launch(const int32_t* perm, int32_t* inv, int64_t n, bool zero_first, hipStream_t s) {
     const int64_t blocks = grid_for((n + 63) / 64);
+    if (zero_first) {
+        // A fill kernel rather than hipMemsetAsync: callers may capture this
+        // sequence into a hipGraph.
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(kBlock), 0, s, inv, n);
+        const int rc = launch_status();
+        if (rc != TROVE_OK) return rc;
+    }
     hipLaunchKernelGGL(k_invert_permutation, dim3((unsigned)blocks), dim3(kBlock), 0, s, perm, inv, n);
     return launch_status();
 }

@@ -14,7 +14,7 @@ int block_write_dispatch(uint32_t* r, int64_t l, int s, hipStream_t st);
 int memcpy_ceiling_launch(const void* src, void* dst, int64_t bytes, hipStream_t s);
 int random_rows_write_launch(void* buf, int64_t n_rows, int row_bytes, const int32_t* row_idx, int64_t n,
                              bool nontemporal, hipStream_t s);
-int invert_permutation_launch(const int32_t* perm, int32_t* inv, int64_t n, hipStream_t s);
+int invert_permutation_launch(const int32_t* perm, int32_t* inv, int64_t n, bool zero_first, hipStream_t s);
 int random_lines_launch(const void* buf, int64_t n_lines, const uint32_t* line_idx, uint32_t* out, int64_t n,
                         hipStream_t s);
 int fill_words_launch(uint32_t* out, int64_t n, uint64_t seed, hipStream_t s);
@@ -213,7 +213,7 @@ int trove_invert_permutation(const int32_t* perm, int32_t* inv, int64_t n, void*
     if (n == 0) return TROVE_OK;
     if (!perm || !inv) return TROVE_ERR_INVALID;
     if (!aligned(perm, 4) || !aligned(inv, 4)) return TROVE_ERR_ALIGN;
-    return invert_permutation_launch(perm, inv, n, static_cast<hipStream_t>(stream));
+    return invert_permutation_launch(perm, inv, n, false, static_cast<hipStream_t>(stream));
 }
 
 int trove_scatter_permutation_inverts(int struct_bytes) {
@@ -231,7 +231,9 @@ int trove_scatter_permutation(const void* src, const int32_t* perm, void* dst, i
     if (!workspace || !auto_scatter_by_inverse(struct_bytes))
         return e->scatter(src, perm, dst, n, TROVE_VARIANT_AUTO, 0, s);
     if (!aligned(workspace, 4)) return TROVE_ERR_ALIGN;
-    const int irc = invert_permutation_launch(perm, workspace, n, s);
+    // Zeroed first: should perm not be a permutation, the rows nothing maps
+    // to gather row 0 and every access stays inside src, dst and workspace.
+    const int irc = invert_permutation_launch(perm, workspace, n, true, s);
     if (irc != TROVE_OK) return irc;
     return e->gather(src, workspace, dst, n, TROVE_VARIANT_AUTO, 0, s);
 }
