@@ -276,8 +276,12 @@ int trove_serve_step(trove_serve* s, int32_t* in_flight);
 int trove_serve_poll(trove_serve* s, uint64_t request_id, int32_t* done, double* latency_s);
 /* Device address of a finished request's n output rows (until released). */
 int trove_serve_output(trove_serve* s, uint64_t request_id, void** dev_out);
-/* Copies a finished request's n rows to `out` (host or device memory),
- * synchronously. */
+/* *n = the request's row count, in any state (the size of the buffer
+ * trove_serve_fetch fills is *n times row_bytes); TROVE_ERR_NOTFOUND for an
+ * unknown or released request. */
+int trove_serve_rows(const trove_serve* s, uint64_t request_id, int64_t* n);
+/* Copies a finished request's n rows to `out` (host or device memory, n *
+ * row_bytes bytes), synchronously. */
 int trove_serve_fetch(trove_serve* s, uint64_t request_id, void* out);
 /* Frees a finished request's rows (the caller's reads must be complete). */
 int trove_serve_release(trove_serve* s, uint64_t request_id);

@@ -268,6 +268,15 @@ def test_serve_copies_host_indices_at_submit(trove, cuda):
     del idx
     srv.drain()
     assert np.array_equal(srv.fetch(rid, 5000), want)
+    # fetch sizes its buffer from the request's own row count.
+    assert srv.rows(rid) == 5000
+    assert np.array_equal(srv.fetch(rid), want)
+    with pytest.raises(ValueError):
+        srv.fetch(rid, 4000)  # a smaller buffer than the request's rows
+    srv.release(rid)
+    with pytest.raises(T.TroveError) as e:
+        srv.rows(rid)
+    assert e.value.code == T.ERR_NOTFOUND
     srv.close()
 
 

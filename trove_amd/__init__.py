@@ -130,6 +130,7 @@ _SIGNATURES = {
     "trove_serve_poll": ([_vp, _u64, _vp, _vp], _int),
     "trove_serve_output": ([_vp, _u64, _vp], _int),
     "trove_serve_release": ([_vp, _u64], _int),
+    "trove_serve_rows": ([_vp, _u64, _vp], _int),
     "trove_serve_fetch": ([_vp, _u64, _vp], _int),
     "trove_serve_drain": ([_vp], _int),
     "trove_serve_stats": ([_vp, _vp, _vp, _vp, _vp, _vp], _int),
@@ -456,11 +457,21 @@ class Server:
         _check(lib().trove_serve_output(self._h, rid, ctypes.byref(p)), "trove_serve_output")
         return p.value or 0
 
-    def fetch(self, rid: int, n: int):
-        """Host copy (uint8, n * row_bytes) of a finished request's rows."""
+    def rows(self, rid: int) -> int:
+        """The request's row count."""
+        n = ctypes.c_int64()
+        _check(lib().trove_serve_rows(self._h, rid, ctypes.byref(n)), "trove_serve_rows")
+        return n.value
+
+    def fetch(self, rid: int, n=None):
+        """Host copy (uint8, rows * row_bytes) of a finished request's rows;
+        n, when given, must equal the request's row count."""
         import numpy as np
 
-        out = np.empty(n * self.row_bytes, dtype=np.uint8)
+        rows = self.rows(rid)
+        if n is not None and n != rows:
+            raise ValueError(f"request {rid} has {rows} rows, not {n}")
+        out = np.empty(rows * self.row_bytes, dtype=np.uint8)
         _check(lib().trove_serve_fetch(self._h, rid, out.ctypes.data), "trove_serve_fetch")
         return out
 

@@ -412,6 +412,14 @@ int trove_serve_output(trove_serve* s, uint64_t request_id, void** dev_out) {
     return TROVE_OK;
 }
 
+int trove_serve_rows(const trove_serve* s, uint64_t request_id, int64_t* n) {
+    if (!s || !n) return TROVE_ERR_INVALID;
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
+    *n = it->second.n;
+    return TROVE_OK;
+}
+
 int trove_serve_fetch(trove_serve* s, uint64_t request_id, void* out) {
     if (!s || !out) return TROVE_ERR_INVALID;
     auto it = s->reqs.find(request_id);
