@@ -64,6 +64,12 @@ def parse_args():
     p.add_argument("--traffic-from", nargs=3, default=None, metavar=("FETCH_DIR", "WRITE_DIR", "LEGS_JSON"),
                    help="no GPU: build profiles/traffic.json from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                         "output directories of a --profile-legs run, print the table as markdown")
+    p.add_argument("--trace-from", default=None, metavar="DIR",
+                   help="no GPU: summarise the rocprofv3 --kernel-trace output under DIR as markdown: per "
+                        "kernel and grid size, and the sweep kernels' algorithmic GB/s at N = 2^--n-log2")
+    p.add_argument("--timeline", default=None, metavar="PREFIX",
+                   help="with --trace-from: instead, the dispatches of kernels whose name starts with PREFIX "
+                        "in time order with their durations and the gaps between them")
     p.add_argument("--counters-from", nargs="+", default=None, metavar="DIR",
                    help="no GPU: DIR... LEGS_JSON: print every counter of rocprofv3 --pmc runs of a "
                         "--profile-legs run per leg (mean over the leg's launches) as a markdown table")
@@ -464,6 +470,99 @@ def _per_leg(rows, counter: str, n_legs: int):
     return out
 
 
+
+def _trace_rows(directory: str):
+    """(kernel name, grid threads, start ns, end ns) of every dispatch in the
+    rocprofv3 kernel_trace.csv files under directory, in start order."""
+    import csv
+    import glob
+
+    rows = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", "*kernel_trace.csv"), recursive=True)):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                grid = 1
+                for axis in ("X", "Y", "Z"):
+                    grid *= int(r.get(f"Grid_Size_{axis}") or 1)
+                if "Grid_Size" in r and r["Grid_Size"]:
+                    grid = int(r["Grid_Size"])
+                rows.append((r["Kernel_Name"], grid, int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    if not rows:
+        raise SystemExit(f"no *kernel_trace.csv under {directory}")
+    rows.sort(key=lambda x: x[2])
+    return rows
+
+
+def _short_kernel(name: str) -> str:
+    """A kernel's demangled name without its return type, namespace and
+    parameter list."""
+    name = name.split("(")[0] if "(" in name else name
+    for prefix in ("void ",):
+        if name.startswith(prefix):
+            name = name[len(prefix):]
+    return name.replace("trove_amd::", "").replace("trove::", "").replace("unsigned int", "u32").strip()
+
+
+# Sweep kernels and their algorithmic bytes per struct of S bytes.
+_SWEEP_KERNELS = {"k_copy_shfl": lambda s: 2 * s, "k_gather_tile": lambda s: 2 * s + 4,
+                  "k_gather_shfl": lambda s: 2 * s + 4}
+
+
+def trace_from(directory: str, timeline=None, n_log2: int = 26) -> int:
+    """Markdown summary of a kernel trace: per (kernel, grid size) launches
+    and durations, then the struct-size sweep's kernels (their dispatches at
+    the largest grid of each, the sweep's launches) as algorithmic GB/s at
+    N = 2^n_log2; or, with a timeline prefix, those dispatches in time order
+    with the gap before each (runs split at pauses over 2 ms)."""
+    import re
+    import statistics as st
+
+    rows = _trace_rows(directory)
+    if timeline:
+        sel = [r for r in rows if _short_kernel(r[0]).startswith(timeline)]
+        if not sel:
+            raise SystemExit(f"no dispatch of a kernel starting with {timeline!r}")
+        run, prev_end = 0, None
+        print("| run | step | kernel | grid (threads) | duration us | gap before us |")
+        print("|---|---|---|---|---|---|")
+        step = 0
+        for name, grid, t0, t1 in sel:
+            gap = None if prev_end is None else (t0 - prev_end) / 1e3
+            if gap is not None and gap > 2000:
+                run, step, gap = run + 1, 0, None
+            print(f"| {run} | {step} | `{_short_kernel(name).split('<')[0]}` | {grid} | {(t1 - t0) / 1e3:.1f} | "
+                  f"{'' if gap is None else f'{gap:.1f}'} |")
+            prev_end, step = t1, step + 1
+        return 0
+    groups = {}
+    for name, grid, t0, t1 in rows:
+        groups.setdefault((_short_kernel(name), grid), []).append((t1 - t0) / 1e3)
+    total_us = sum(sum(v) for v in groups.values())
+    print("| kernel | grid (threads) | launches | avg us | min us | max us | total ms | share |")
+    print("|---|---|---|---|---|---|---|---|")
+    for (name, grid), d in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| `{name}` | {grid} | {len(d)} | {st.mean(d):.1f} | {min(d):.1f} | {max(d):.1f} | "
+              f"{sum(d) / 1e3:.2f} | {100 * sum(d) / total_us:.1f}% |")
+    sweep = {}
+    for (name, grid), d in groups.items():
+        m = re.match(r"(k_\w+)<array<u32, (\d+)>", name)
+        if not m or m.group(1) not in _SWEEP_KERNELS:
+            continue
+        key = (m.group(1), int(m.group(2)))
+        if key not in sweep or grid > sweep[key][0]:
+            sweep[key] = (grid, d)
+    if sweep:
+        n = 1 << n_log2
+        print()
+        print(f"Sweep kernels (their dispatches at the largest grid of each), N = 2^{n_log2} structs:")
+        print()
+        print("| kernel | k | grid (threads) | launches | avg us | algorithmic GB/s |")
+        print("|---|---|---|---|---|---|")
+        for (kern, k), (grid, d) in sorted(sweep.items(), key=lambda kv: (kv[0][1], kv[0][0])):
+            gbs = _SWEEP_KERNELS[kern](4 * k) * n / (st.mean(d) * 1e-6) / 1e9
+            print(f"| `{kern}` | {k} | {grid} | {len(d)} | {st.mean(d):.1f} | {gbs:.0f} |")
+    return 0
+
 def counters_from(dirs, legs_json: str) -> int:
     """Per-leg means of every counter found in the counter runs under dirs
     (each a --pmc run of the same --profile-legs program), as markdown."""
@@ -607,6 +706,8 @@ def main():
     args = parse_args()
     if args.traffic_from:
         return traffic_from(*args.traffic_from)
+    if args.trace_from:
+        return trace_from(args.trace_from, args.timeline, args.n_log2)
     if args.counters_from:
         if len(args.counters_from) < 2:
             raise SystemExit("--counters-from needs at least one counter directory and the legs JSON")

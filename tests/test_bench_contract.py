@@ -106,6 +106,38 @@ def test_traffic_from_attributes_dispatches_to_legs(tmp_path):
         bench.traffic_from(str(tmp_path / "f"), str(tmp_path / "w"), str(tmp_path / "legs.json"), str(out))
 
 
+def test_trace_from_summarises_kernel_traces(tmp_path):
+    import contextlib
+    import io
+
+    import bench
+
+    d = tmp_path / "t" / "host"
+    os.makedirs(d)
+    g15 = "void trove_amd::k_gather_shfl<trove::array<unsigned int, 15> >(trove::array<unsigned int, 15> const*, int const*, trove::array<unsigned int, 15>*, long)"
+    rows = [(g15, 67108864, 0, 3000000), (g15, 67108864, 4000000, 7000000), (g15, 4096, 8000000, 8001000),
+            ("void trove_amd::k_batch_gather(trove_amd::job_desc const*, int, long, int*)", 8192, 20000000, 20100000),
+            ("void trove_amd::k_batch_gather_pack(trove_amd::job_pack)", 4096, 20110000, 20150000)]
+    with open(d / "1_kernel_trace.csv", "w") as f:
+        f.write("Kernel_Name,Grid_Size_X,Grid_Size_Y,Grid_Size_Z,Start_Timestamp,End_Timestamp\n")
+        for name, grid, t0, t1 in rows:
+            f.write(f'"{name}",{grid},1,1,{t0},{t1}\n')
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        assert bench.trace_from(str(tmp_path / "t"), None, 26) == 0
+    out = buf.getvalue().splitlines()
+    # Per kernel and grid; the sweep table takes the largest grid: 2^26 rows
+    # of 124 algorithmic bytes in 3,000 us.
+    assert "| `k_gather_shfl<array<u32, 15> >` | 67108864 | 2 | 3000.0 | 3000.0 | 3000.0 | 6.00 | 97.7% |" in out
+    assert any(line.startswith("| `k_gather_shfl` | 15 | 67108864 | 2 | 3000.0 | 2774 |") for line in out)
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        assert bench.trace_from(str(tmp_path / "t"), "k_batch", 26) == 0
+    out = buf.getvalue().splitlines()
+    assert out[2] == "| 0 | 0 | `k_batch_gather` | 8192 | 100.0 |  |"
+    assert out[3] == "| 0 | 1 | `k_batch_gather_pack` | 4096 | 40.0 | 10.0 |"
+
+
 @pytest.mark.gpu
 def test_small_bench_run_keeps_the_contract(cuda):
     p = _bench("--k", "1,5", "--n-log2", "18", "--steps", "2", "--warmup", "0", "--no-cpu", "--jobs", "64",
