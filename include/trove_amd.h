@@ -207,7 +207,10 @@ typedef struct trove_job {
     int32_t reserved;     /* must be 0 */
 } trove_job;
 
-typedef struct trove_batch trove_batch; /* opaque plan */
+/* Opaque plan. Runs of one plan may be issued from several host threads
+ * (the graph of TROVE_BATCH_GRAPH is captured once, under a lock); they
+ * write the same outputs, so runs that overlap on the device race. */
+typedef struct trove_batch trove_batch;
 
 #define TROVE_BATCH_LAUNCH 0   /* one launch, one wavefront per work item of every job */
 #define TROVE_BATCH_PERSISTENT TROVE_BATCH_LAUNCH /* the same mode's earlier name */

@@ -101,6 +101,35 @@ def test_batch_graph_replays(trove, cuda):
     b.close()
 
 
+def test_batch_graph_first_run_from_two_threads(trove, cuda):
+    # The first TROVE_BATCH_GRAPH runs of one plan from two host threads at
+    # once: the graph is captured once (under the plan's lock) and both
+    # replays, ordered on the default stream, write the right outputs.
+    import threading
+
+    jobs = _jobs(64, 12)
+    b = _device_jobs(jobs)
+    errors = []
+    start = threading.Barrier(2)
+
+    def run():
+        try:
+            start.wait()
+            b.run("graph")
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=run) for _ in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    torch.cuda.synchronize()
+    _check(jobs, "graph from two threads")
+    b.close()
+
+
 def test_batch_rejects_bad_jobs(trove, cuda):
     src = torch.zeros(1024, dtype=torch.uint8, device=cuda)
     idx = torch.zeros(16, dtype=torch.int32, device=cuda)

@@ -221,6 +221,7 @@ struct trove_batch {
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    std::mutex capture_mutex;  // the lazy capture of TROVE_BATCH_GRAPH, once per handle
 };
 
 static int batch_launch(trove_batch* b, hipStream_t s) {
@@ -371,11 +372,16 @@ int trove_batch_run(trove_batch* b, int mode, void* stream) {
     if (mode == TROVE_BATCH_RESIDENT)
         return launch_job_table_resident(b->d_table, (int)b->table.size(), b->total_items, b->lds, nullptr, s);
     if (mode == TROVE_BATCH_GRAPH) {
-        if (!b->graph_exec) {
-            const int rc = capture_graph(b);
-            if (rc != TROVE_OK) return rc;
+        hipGraphExec_t exec = nullptr;
+        {
+            std::lock_guard<std::mutex> lock(b->capture_mutex);
+            if (!b->graph_exec) {
+                const int rc = capture_graph(b);
+                if (rc != TROVE_OK) return rc;
+            }
+            exec = b->graph_exec;
         }
-        return hip_status(hipGraphLaunch(b->graph_exec, s));
+        return hip_status(hipGraphLaunch(exec, s));
     }
     if (mode == TROVE_BATCH_LOOPED) {
         for (const job_desc& d : b->table) {
