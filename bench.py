@@ -197,7 +197,7 @@ def cpu_baseline(ks, n_log2, seed):
         total_bytes += n * (2 * s) + n * (2 * s + 4)
     threads = oracle.lib().oracle_max_threads()
     return {
-        "value": round(total_bytes / total_s / 1e9, 2),
+        "value": float(f"{total_bytes / total_s / 1e9:.4g}"),  # 4 significant digits, also for tiny samples
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
