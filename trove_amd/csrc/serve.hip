@@ -40,7 +40,14 @@ namespace trove_amd {
 
 namespace {
 
-constexpr int kInflight = 4;
+// Steps queued on the compute stream at most. Fewer steps in flight make
+// each step take more of the requests that arrived meanwhile instead of
+// queueing behind small steps already launched. Measured on MI355X (2^24 x
+// 32-B table, Zipf requests, interleaved runs): against four, two steps
+// lower p50 at 1 / 1.5 M requests/s from 0.054-0.060 / 0.083-0.098 ms to
+// 0.033-0.035 / 0.046-0.057 ms with saturation unchanged (1.99-2.08 M
+// requests/s); one step costs 2-3% of saturation.
+constexpr int kInflight = 2;
 
 double now_s() {
     using clk = std::chrono::steady_clock;
