@@ -91,16 +91,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
 
 
+# The translation units whose kernels profiles/traffic.json measures (copy,
+# gather, scatter, store / load, block interface, ceilings, permutation
+# inversion) and the headers they include; the batch engine and the server
+# (batch.hip, batch.h, serve.hip) have no leg in that table.
+TRAFFIC_SOURCES = ("common.h", "gather_body.h", "copy.hip", "gather.hip", "block.hip", "capi.hip")
+
+
 def kernel_sources_sha256() -> str:
-    """Hash of the kernel sources (trove_amd/csrc, include/): the stamp that
-    ties profiles/traffic.json to the kernels it was measured on."""
+    """Hash of the sources of the kernels profiles/traffic.json measures
+    (TRAFFIC_SOURCES under trove_amd/csrc, and include/): the stamp that ties
+    the table to the kernels it was measured on."""
     import hashlib
 
     h = hashlib.sha256()
-    files = []
-    for top in (os.path.join(ROOT, "trove_amd", "csrc"), os.path.join(ROOT, "include")):
-        for d, _, names in os.walk(top):
-            files += [os.path.join(d, f) for f in names if f.endswith((".hip", ".h", ".cpp"))]
+    files = [os.path.join(ROOT, "trove_amd", "csrc", f) for f in TRAFFIC_SOURCES]
+    for d, _, names in os.walk(os.path.join(ROOT, "include")):
+        files += [os.path.join(d, f) for f in names if f.endswith((".hip", ".h", ".cpp"))]
     for f in sorted(files, key=lambda x: os.path.relpath(x, ROOT)):
         h.update(os.path.relpath(f, ROOT).encode())
         with open(f, "rb") as fh:

@@ -295,10 +295,14 @@ int trove_serve_stats(const trove_serve* s, int64_t* completed, int64_t* indices
 /* Replays an arrival trace in real time: request i (idx[i], n[i] rows) is
  * submitted at arrival_s[i] seconds after the start (ascending); returns
  * when all are done, with latency_s[i] = completion - scheduled arrival
- * (-1 for a request rejected for a device index outside the table).
- * release_on_done frees each request's rows as soon as it completes. */
+ * (-1 for a request rejected for a device index outside the table) and,
+ * when request_ids is not NULL, request_ids[i] = request i's id (for
+ * trove_serve_output / _fetch / _release when its rows are kept).
+ * release_on_done frees each request's rows as soon as it completes.
+ * Request ids are opaque: no order or starting value is promised. */
 int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
-                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s);
+                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s,
+                       uint64_t* request_ids);
 int trove_serve_destroy(trove_serve* s);
 
 /* Deterministic inputs (host memory; no GPU needed).

@@ -196,13 +196,13 @@ def test_serve_device_indices_and_replay(trove, cuda):
     # The budget holds every output (none is released during the replay).
     assert sum(r.size for r in reqs) * rb <= 1 << 23
     srv = T.Server(table, n_rows, rb, budget_bytes=1 << 23, max_slots=16, slice_indices=1024)
-    lat = srv.replay(arrivals, [d.data_ptr() for d in d_reqs], [r.size for r in reqs], on_device=True,
-                     release_on_done=False)
+    lat, ids = srv.replay(arrivals, [d.data_ptr() for d in d_reqs], [r.size for r in reqs], on_device=True,
+                          release_on_done=False, return_ids=True)
     assert np.all(lat >= 0) and np.all(np.isfinite(lat))
     assert srv.stats()["completed"] == len(reqs)
-    # Request ids are assigned in submission order starting at 1.
+    assert len(set(ids)) == len(reqs) and 0 not in ids
     for k, r in enumerate(reqs):
-        assert np.array_equal(srv.fetch(k + 1, r.size), oracle.gather(host_table, r, rb)), k
+        assert np.array_equal(srv.fetch(ids[k], r.size), oracle.gather(host_table, r, rb)), k
     srv.close()
 
 

@@ -134,7 +134,7 @@ _SIGNATURES = {
     "trove_serve_fetch": ([_vp, _u64, _vp], _int),
     "trove_serve_drain": ([_vp], _int),
     "trove_serve_stats": ([_vp, _vp, _vp, _vp, _vp, _vp], _int),
-    "trove_serve_replay": ([_vp, ctypes.c_int32, _vp, _vp, _vp, _int, _int, _vp], _int),
+    "trove_serve_replay": ([_vp, ctypes.c_int32, _vp, _vp, _vp, _int, _int, _vp, _vp], _int),
     "trove_serve_destroy": ([_vp], _int),
 }
 
@@ -489,8 +489,10 @@ class Server:
         return {"completed": c.value, "indices_done": i.value, "steps": st.value, "queued": q.value,
                 "active": a.value}
 
-    def replay(self, arrival_s, idx_ptrs, counts, on_device: bool, release_on_done: bool = True):
-        """Real-time replay of an arrival trace; returns per-request latency (s)."""
+    def replay(self, arrival_s, idx_ptrs, counts, on_device: bool, release_on_done: bool = True,
+               return_ids: bool = False):
+        """Real-time replay of an arrival trace; returns per-request latency
+        (s), or (latency, request ids) with return_ids."""
         import numpy as np
 
         n = len(counts)
@@ -500,9 +502,11 @@ class Server:
         ptrs = np.ascontiguousarray(idx_ptrs, dtype=np.uint64)
         cnt = np.ascontiguousarray(counts, dtype=np.int64)
         lat = np.zeros(n, dtype=np.float64)
+        ids = np.zeros(n, dtype=np.uint64)
         _check(lib().trove_serve_replay(self._h, n, arr.ctypes.data, ptrs.ctypes.data, cnt.ctypes.data,
-                                        int(on_device), int(release_on_done), lat.ctypes.data), "trove_serve_replay")
-        return lat
+                                        int(on_device), int(release_on_done), lat.ctypes.data, ids.ctypes.data),
+               "trove_serve_replay")
+        return (lat, [int(i) for i in ids]) if return_ids else lat
 
     def close(self) -> None:
         if self._h:

@@ -33,7 +33,6 @@
 #include <deque>
 #include <map>
 #include <new>
-#include <unordered_map>
 #include <vector>
 
 namespace trove_amd {
@@ -96,6 +95,7 @@ struct request {
     const int32_t* idx = nullptr;  // caller's device indices, or host_idx.data()
     std::vector<int32_t> host_idx;  // copy of host indices, taken at submission
     bool idx_on_device = false;
+    bool live = false;       // the slot holds a request (request_table)
     bool bad_index = false;  // a step flagged an index outside the table
     int64_t n = 0;
     int64_t rows = 0;       // arena rows reserved (>= 1)
@@ -104,6 +104,61 @@ struct request {
     int state = QUEUED;
     double t_submit = 0, t_done = 0;
     const int32_t* src_idx = nullptr;  // indices the kernel reads (caller's or arena copy)
+};
+
+// Requests live in the slots of one vector, and a released request's slot is
+// reused. A request id is (generation << 32) | slot: ids stay unique across
+// reuse (until one slot has been reused 2^32 times), and a lookup is an index
+// plus an id check, with no allocation per request once the vector has grown
+// to the number of live requests. (Measured on MI355X's host: with a hash map
+// of requests, submitting and completing a request cost about 1 us of host
+// time, enough to saturate the host loop near 1 M requests/s on a slow core.)
+class request_table {
+  public:
+    request* find(uint64_t id) {
+        const uint64_t k = id & 0xffffffffu;
+        return k < slots_.size() && slots_[k].live && slots_[k].id == id ? &slots_[k] : nullptr;
+    }
+    const request* find(uint64_t id) const { return const_cast<request_table*>(this)->find(id); }
+    // The request of an id known to be live.
+    request& operator[](uint64_t id) { return slots_[id & 0xffffffffu]; }
+    // A fresh request in a free slot. May move the slots: references to
+    // requests do not survive it.
+    request& create() {
+        uint32_t k;
+        if (!free_.empty()) {
+            k = free_.back();
+            free_.pop_back();
+        } else {
+            k = (uint32_t)slots_.size();
+            slots_.emplace_back();
+        }
+        request& r = slots_[k];
+        uint32_t gen = (uint32_t)(r.id >> 32) + 1;
+        if (gen == 0) gen = 1;
+        r = request();
+        r.id = ((uint64_t)gen << 32) | k;
+        r.live = true;
+        return r;
+    }
+    void erase(request& r) {
+        r.live = false;
+        std::vector<int32_t>().swap(r.host_idx);
+        free_.push_back((uint32_t)(r.id & 0xffffffffu));
+    }
+    // Room for n more requests without moving the slots.
+    void reserve_more(size_t n) {
+        const size_t want = slots_.size() - free_.size() + n;
+        if (want > slots_.capacity()) slots_.reserve(want);
+        free_.reserve(slots_.capacity());
+    }
+    // Upper bound (exclusive) of the slot numbers of ids created before the
+    // slots next grow beyond their reserved capacity.
+    size_t slot_bound() const { return slots_.capacity(); }
+
+  private:
+    std::vector<request> slots_;
+    std::vector<uint32_t> free_;
 };
 
 struct step_rec {
@@ -135,8 +190,8 @@ struct trove_serve {
     hipEvent_t upload_done = nullptr;  // last host-index upload pass
     std::deque<uint64_t> queue;   // QUEUED, FIFO
     std::vector<uint64_t> active;  // ACTIVE, admission order
-    std::unordered_map<uint64_t, request> reqs;
-    uint64_t next_id = 1;
+    std::vector<uint64_t> still_active;  // launch_step's next `active` (kept for its capacity)
+    request_table reqs;
     // step ring
     step_rec steps[kInflight];
     job_desc* h_jobs[kInflight] = {};  // pinned
@@ -238,8 +293,8 @@ static int launch_step(trove_serve* s) {
     const int32_t last_row = (int32_t)(s->n_rows - 1);
     int nj = 0;
     int64_t items = 0;
-    std::vector<uint64_t> still_active;
-    still_active.reserve(s->active.size());
+    std::vector<uint64_t>& still_active = s->still_active;
+    still_active.clear();
     for (uint64_t id : s->active) {
         request& r = s->reqs[id];
         const int64_t left = r.n - r.scheduled;
@@ -367,8 +422,7 @@ int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on
     } else if (!aligned(idx, 4)) {
         return TROVE_ERR_ALIGN;
     }
-    request r;
-    r.id = s->next_id++;
+    request& r = s->reqs.create();
     r.idx_on_device = idx_on_device != 0;
     if (r.idx_on_device) {
         r.idx = idx;
@@ -376,17 +430,14 @@ int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on
         // Copied right after the check: later changes to the caller's array
         // (or its release) cannot reach the gather.
         r.host_idx.assign(idx, idx + n);
+        r.idx = r.host_idx.data();
     }
     r.n = n;
     r.rows = n > 0 ? n : 1;
     r.t_submit = now_s();
     *request_id = r.id;
-    auto& slot = s->reqs[r.id];
-    slot = std::move(r);
-    if (!slot.idx_on_device) slot.idx = slot.host_idx.data();
     if (n == 0) {
-        finish_empty(s, slot);
-        slot.row0 = -1;
+        finish_empty(s, r);
         return TROVE_OK;
     }
     s->queue.push_back(r.id);
@@ -402,49 +453,49 @@ int trove_serve_step(trove_serve* s, int32_t* in_flight) {
 
 int trove_serve_poll(trove_serve* s, uint64_t request_id, int32_t* done, double* latency_s) {
     if (!s || !done) return TROVE_ERR_INVALID;
-    auto it = s->reqs.find(request_id);
-    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
-    *done = it->second.state == DONE;
-    if (latency_s) *latency_s = it->second.state == DONE ? it->second.t_done - it->second.t_submit : -1.0;
-    return it->second.state == DONE && it->second.bad_index ? TROVE_ERR_RANGE : TROVE_OK;
+    const request* r = s->reqs.find(request_id);
+    if (!r) return TROVE_ERR_NOTFOUND;
+    *done = r->state == DONE;
+    if (latency_s) *latency_s = r->state == DONE ? r->t_done - r->t_submit : -1.0;
+    return r->state == DONE && r->bad_index ? TROVE_ERR_RANGE : TROVE_OK;
 }
 
 int trove_serve_output(trove_serve* s, uint64_t request_id, void** dev_out) {
     if (!s || !dev_out) return TROVE_ERR_INVALID;
-    auto it = s->reqs.find(request_id);
-    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
-    if (it->second.state != DONE) return TROVE_ERR_AGAIN;
-    if (it->second.bad_index) return TROVE_ERR_RANGE;
-    *dev_out = it->second.row0 >= 0 ? s->out_arena + it->second.row0 * (int64_t)s->row_bytes : nullptr;
+    const request* r = s->reqs.find(request_id);
+    if (!r) return TROVE_ERR_NOTFOUND;
+    if (r->state != DONE) return TROVE_ERR_AGAIN;
+    if (r->bad_index) return TROVE_ERR_RANGE;
+    *dev_out = r->row0 >= 0 ? s->out_arena + r->row0 * (int64_t)s->row_bytes : nullptr;
     return TROVE_OK;
 }
 
 int trove_serve_rows(const trove_serve* s, uint64_t request_id, int64_t* n) {
     if (!s || !n) return TROVE_ERR_INVALID;
-    auto it = s->reqs.find(request_id);
-    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
-    *n = it->second.n;
+    const request* r = s->reqs.find(request_id);
+    if (!r) return TROVE_ERR_NOTFOUND;
+    *n = r->n;
     return TROVE_OK;
 }
 
 int trove_serve_fetch(trove_serve* s, uint64_t request_id, void* out) {
     if (!s || !out) return TROVE_ERR_INVALID;
-    auto it = s->reqs.find(request_id);
-    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
-    if (it->second.state != DONE) return TROVE_ERR_AGAIN;
-    if (it->second.bad_index) return TROVE_ERR_RANGE;
-    if (it->second.n == 0) return TROVE_OK;
-    const void* src = s->out_arena + it->second.row0 * (int64_t)s->row_bytes;
-    return hip_status(hipMemcpy(out, src, (size_t)(it->second.n * s->row_bytes), hipMemcpyDefault));
+    const request* r = s->reqs.find(request_id);
+    if (!r) return TROVE_ERR_NOTFOUND;
+    if (r->state != DONE) return TROVE_ERR_AGAIN;
+    if (r->bad_index) return TROVE_ERR_RANGE;
+    if (r->n == 0) return TROVE_OK;
+    const void* src = s->out_arena + r->row0 * (int64_t)s->row_bytes;
+    return hip_status(hipMemcpy(out, src, (size_t)(r->n * s->row_bytes), hipMemcpyDefault));
 }
 
 int trove_serve_release(trove_serve* s, uint64_t request_id) {
     if (!s) return TROVE_ERR_INVALID;
-    auto it = s->reqs.find(request_id);
-    if (it == s->reqs.end()) return TROVE_ERR_NOTFOUND;
-    if (it->second.state != DONE) return TROVE_ERR_AGAIN;
-    if (it->second.row0 >= 0) s->arena->release(it->second.row0, it->second.rows);
-    s->reqs.erase(it);
+    request* r = s->reqs.find(request_id);
+    if (!r) return TROVE_ERR_NOTFOUND;
+    if (r->state != DONE) return TROVE_ERR_AGAIN;
+    if (r->row0 >= 0) s->arena->release(r->row0, r->rows);
+    s->reqs.erase(*r);
     return TROVE_OK;
 }
 
@@ -479,11 +530,14 @@ int trove_serve_stats(const trove_serve* s, int64_t* completed, int64_t* indices
 // The replay loop proper; trove_serve_replay brackets it with the recording
 // of completions in newly_done.
 static int replay_loop(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
-                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s) {
-    std::vector<uint64_t> ids(n_req, 0);
-    std::unordered_map<uint64_t, int32_t> slot_of;
-    slot_of.reserve((size_t)n_req * 2);
-    s->reqs.reserve(s->reqs.size() + (size_t)n_req);
+                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s,
+                       uint64_t* request_ids) {
+    std::vector<uint64_t> own_ids(request_ids ? 0 : n_req, 0);
+    uint64_t* ids = request_ids ? request_ids : own_ids.data();
+    // Arrival index of the request in each table slot (-1: none of this
+    // replay's); the reservation keeps every slot number below slot_bound().
+    s->reqs.reserve_more((size_t)n_req);
+    std::vector<int32_t> arrival_of(s->reqs.slot_bound(), -1);
     int32_t next = 0, done = 0;
     // A rejected request (an index outside the table) reports latency -1.
     auto latency_of = [](const request& r) { return r.bad_index ? -1.0 : r.t_done - r.t_submit; };
@@ -506,7 +560,7 @@ static int replay_loop(trove_serve* s, int32_t n_req, const double* arrival_s, c
                 ++done;
                 if (release_on_done) trove_serve_release(s, ids[next]);
             } else {
-                slot_of[ids[next]] = next;
+                arrival_of[ids[next] & 0xffffffffu] = next;
             }
             ++next;
         }
@@ -518,11 +572,11 @@ static int replay_loop(trove_serve* s, int32_t n_req, const double* arrival_s, c
             return TROVE_ERR_FULL;
         }
         for (uint64_t id : s->newly_done) {
-            auto it = slot_of.find(id);
-            if (it == slot_of.end()) continue;
-            latency_s[it->second] = latency_of(s->reqs[id]);
+            int32_t& a = arrival_of[id & 0xffffffffu];
+            if (a < 0 || ids[a] != id) continue;  // submitted before this replay
+            latency_s[a] = latency_of(s->reqs[id]);
             ++done;
-            slot_of.erase(it);
+            a = -1;
             if (release_on_done) trove_serve_release(s, id);
         }
         s->newly_done.clear();
@@ -531,11 +585,12 @@ static int replay_loop(trove_serve* s, int32_t n_req, const double* arrival_s, c
 }
 
 int trove_serve_replay(trove_serve* s, int32_t n_req, const double* arrival_s, const int32_t* const* idx,
-                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s) {
+                       const int64_t* n, int idx_on_device, int release_on_done, double* latency_s,
+                       uint64_t* request_ids) {
     if (!s || n_req < 0 || (n_req > 0 && (!arrival_s || !idx || !n || !latency_s))) return TROVE_ERR_INVALID;
     s->newly_done.clear();
     s->replaying = true;
-    const int rc = replay_loop(s, n_req, arrival_s, idx, n, idx_on_device, release_on_done, latency_s);
+    const int rc = replay_loop(s, n_req, arrival_s, idx, n, idx_on_device, release_on_done, latency_s, request_ids);
     s->replaying = false;
     s->newly_done.clear();
     return rc;
