@@ -354,7 +354,7 @@ __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value, 
     } else {
         tile_gather_words<n, Policy>(src, p0, o, id, first, v);
     }
-    r2c_words<n, L::m, L::W>(v, id, first);
+    r2c_words<n, L::m, L::W, tx_by_plane<L::m, L::W>()>(v, id, first);
     T out;
     words_to_struct(v, out);
     return out;
@@ -376,7 +376,7 @@ __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value>:
     const tile_offset o = offset_from(dest, p0);
     uint32_t v[L::m * L::W];
     struct_to_words(data, v);
-    c2r_words<n, L::m, L::W>(v, id, first);
+    c2r_words<n, L::m, L::W, tx_by_plane<L::m, L::W>()>(v, id, first);
     if (tile_all<n>(o.exact && o.delta == id)) {
         tile_store_words<n, T, Policy>(p0, id, v);
     } else {

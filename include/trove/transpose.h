@@ -26,47 +26,79 @@ namespace detail {
 
 // R2C on m elements of W dwords each (flat in v), tile width n.
 // `id` is the rank in the tile, `first` the wavefront lane of rank 0.
-template <int n, int m, int W>
+// Every step acts on the W dwords of an element alike, so with ByPlane the
+// transposition runs one dword plane (dword x of every element) at a time:
+// a barrel-shifter stage then keeps m temporaries alive instead of m * W,
+// at the price of less independent work between the exchanges.
+template <int n, int m, int W, bool ByPlane = false>
 __device__ __forceinline__ void r2c_words(uint32_t (&v)[m * W], int id, int first, const int* partner,
                                           int lane_rot) {
     typedef tx_schedule<n, m> S;
-    uint32_t t[m * W];
+    constexpr int PW = ByPlane ? 1 : W;  // dwords of an element per pass
 #pragma unroll
-    for (int b = 0; b < m; ++b)
+    for (int x0 = 0; x0 < W; x0 += PW) {
+        uint32_t t[m * PW];
 #pragma unroll
-        for (int x = 0; x < W; ++x) t[S::P(b) * W + x] = v[b * W + x];
-    rotate_words_left<m, W>(t, lane_rot);
+        for (int b = 0; b < m; ++b)
 #pragma unroll
-    for (int r = 0; r < m; ++r) {
-        const int addr = (first + partner[r]) << 2;
+            for (int x = 0; x < PW; ++x) t[S::P(b) * PW + x] = v[b * W + x0 + x];
+        rotate_words_left<m, PW>(t, lane_rot);
 #pragma unroll
-        for (int x = 0; x < W; ++x) t[r * W + x] = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)t[r * W + x]);
+        for (int r = 0; r < m; ++r) {
+            const int addr = (first + partner[r]) << 2;
+#pragma unroll
+            for (int x = 0; x < PW; ++x)
+                t[r * PW + x] = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)t[r * PW + x]);
+        }
+        if constexpr (S::has_row_rotation) rotate_words_left<m, PW>(t, S::h(id));
+#pragma unroll
+        for (int r = 0; r < m; ++r)
+#pragma unroll
+            for (int x = 0; x < PW; ++x) v[r * W + x0 + x] = t[r * PW + x];
     }
-    if constexpr (S::has_row_rotation) rotate_words_left<m, W>(t, S::h(id));
-#pragma unroll
-    for (int i = 0; i < m * W; ++i) v[i] = t[i];
 }
 
 // C2R on m elements of W dwords each: the inverse of r2c_words.
-template <int n, int m, int W>
+template <int n, int m, int W, bool ByPlane = false>
 __device__ __forceinline__ void c2r_words(uint32_t (&v)[m * W], int id, int first, const int* partner,
                                           int lane_rot) {
     typedef tx_schedule<n, m> S;
-    if constexpr (S::has_row_rotation) rotate_words_left<m, W>(v, S::unrotate_row(id));
+    constexpr int PW = ByPlane ? 1 : W;
 #pragma unroll
-    for (int r = 0; r < m; ++r) {
-        const int addr = (first + partner[r]) << 2;
+    for (int x0 = 0; x0 < W; x0 += PW) {
+        uint32_t t[m * PW];
 #pragma unroll
-        for (int x = 0; x < W; ++x) v[r * W + x] = (uint32_t)__builtin_amdgcn_ds_permute(addr, (int)v[r * W + x]);
+        for (int r = 0; r < m; ++r)
+#pragma unroll
+            for (int x = 0; x < PW; ++x) t[r * PW + x] = v[r * W + x0 + x];
+        if constexpr (S::has_row_rotation) rotate_words_left<m, PW>(t, S::unrotate_row(id));
+#pragma unroll
+        for (int r = 0; r < m; ++r) {
+            const int addr = (first + partner[r]) << 2;
+#pragma unroll
+            for (int x = 0; x < PW; ++x)
+                t[r * PW + x] = (uint32_t)__builtin_amdgcn_ds_permute(addr, (int)t[r * PW + x]);
+        }
+        rotate_words_left<m, PW>(t, lane_rot);
+#pragma unroll
+        for (int b = 0; b < m; ++b)
+#pragma unroll
+            for (int x = 0; x < PW; ++x) v[b * W + x0 + x] = t[S::P(b) * PW + x];
     }
-    rotate_words_left<m, W>(v, lane_rot);
-    uint32_t t[m * W];
-#pragma unroll
-    for (int b = 0; b < m; ++b)
-#pragma unroll
-        for (int x = 0; x < W; ++x) t[b * W + x] = v[S::P(b) * W + x];
-#pragma unroll
-    for (int i = 0; i < m * W; ++i) v[i] = t[i];
+}
+
+// Whether the coalesced_ptr path (load_dispatch / store_dispatch) transposes
+// one dword plane at a time: for structs of 16-byte words holding 84 to 112
+// dwords (336 to 448 bytes), where the full-width form's second copy of the
+// struct pushes the kernel past the register budget of two wavefronts per
+// SIMD and one plane does not. Measured on MI355X at 384 bytes (2^22
+// structs): coalesced_ptr gather 3.83 -> 5.41 TB/s (274 -> 255 registers,
+// one -> two wavefronts per SIMD), scatter 4.23 -> 4.90. At 96-256 and 512
+// bytes the planes changed the gathers by -4% to +1%, at 768 bytes by -7%,
+// and the warp-contiguous copies by -3% to +2%, so they keep the full width.
+template <int m, int W>
+__host__ __device__ constexpr bool tx_by_plane() {
+    return W == 4 && m * W > 80 && m * W <= 112;
 }
 
 // Square tiles (m == n, a power of two): the transposition is the plain
@@ -138,25 +170,25 @@ __device__ __forceinline__ void partner_table(int id, int (&partner)[m]) {
 // Self-contained forms: derive the per-lane tables from the tile rank.
 // Square tiles take the butterfly transpose; every other shape the
 // partner schedule.
-template <int n, int m, int W>
+template <int n, int m, int W, bool ByPlane = false>
 __device__ __forceinline__ void r2c_words(uint32_t (&v)[m * W], int id, int first) {
     if constexpr (m == n) {
         square_transpose<n, W>(v, first + id);
     } else {
         int partner[m];
         partner_table<n, m>(id, partner);
-        r2c_words<n, m, W>(v, id, first, partner, tx_schedule<n, m>::unrotate_lane(id));
+        r2c_words<n, m, W, ByPlane>(v, id, first, partner, tx_schedule<n, m>::unrotate_lane(id));
     }
 }
 
-template <int n, int m, int W>
+template <int n, int m, int W, bool ByPlane = false>
 __device__ __forceinline__ void c2r_words(uint32_t (&v)[m * W], int id, int first) {
     if constexpr (m == n) {
         square_transpose<n, W>(v, first + id);
     } else {
         int partner[m];
         partner_table<n, m>(id, partner);
-        c2r_words<n, m, W>(v, id, first, partner, id % m);
+        c2r_words<n, m, W, ByPlane>(v, id, first, partner, id % m);
     }
 }
 
