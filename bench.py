@@ -553,7 +553,9 @@ def trace_from(directory: str, timeline=None, n_log2: int = 26) -> int:
     sweep = {}
     for (name, grid), d in groups.items():
         m = re.match(r"(k_\w+)<array<u32, (\d+)>", name)
-        if not m or m.group(1) not in _SWEEP_KERNELS:
+        # The sweep's sizes are k = 1..16 words; the large-struct legs share
+        # the kernel names but run at other N.
+        if not m or m.group(1) not in _SWEEP_KERNELS or int(m.group(2)) > 16:
             continue
         key = (m.group(1), int(m.group(2)))
         if key not in sweep or grid > sweep[key][0]:
@@ -619,7 +621,8 @@ def traffic_from(fetch_dir: str, write_dir: str, legs_json: str, out_path: str =
     f_fac = each_way / (c[1] / c[0] * 1024)
     w_fac = each_way / (c[2] / c[0] * 1024)
     try:
-        commit = subprocess.run(["git", "-C", ROOT, "log", "-1", "--format=%h", "--", "trove_amd/csrc", "include"],
+        paths = [f"trove_amd/csrc/{f}" for f in TRAFFIC_SOURCES] + ["include"]
+        commit = subprocess.run(["git", "-C", ROOT, "log", "-1", "--format=%h", "--", *paths],
                                 capture_output=True, text=True, timeout=20).stdout.strip() or None
     except (OSError, subprocess.SubprocessError):
         commit = None

@@ -309,6 +309,34 @@ def test_serve_copies_host_indices_at_submit(trove, cuda):
     srv.close()
 
 
+def test_serve_request_ids_stay_unique_across_slot_reuse(trove, cuda):
+    # A released request's table slot is reused by the next submission; the
+    # old id must then name nothing, and every live id its own request.
+    n_rows, rb = 1 << 12, 16
+    host_table, table = _table(n_rows, rb, 23)
+    srv = T.Server(table, n_rows, rb, budget_bytes=1 << 20, max_slots=4, slice_indices=1024)
+    rng = np.random.default_rng(23)
+    seen, live = set(), {}
+    for round_ in range(6):
+        reqs = [rng.integers(0, n_rows, size=int(rng.integers(1, 400)), dtype=np.int32) for _ in range(3)]
+        ids = [srv.submit(r, r.size, on_device=False) for r in reqs]
+        assert not (set(ids) & seen), round_  # never an id handed out before
+        seen.update(ids)
+        srv.drain()
+        for rid, r in zip(ids, reqs):
+            live[rid] = r
+        # Release all but the newest request, so later submissions reuse slots.
+        for rid in list(live)[:-1]:
+            srv.release(rid)
+            with pytest.raises(T.TroveError) as e:
+                srv.rows(rid)
+            assert e.value.code == T.ERR_NOTFOUND
+            del live[rid]
+        for rid, r in live.items():
+            assert np.array_equal(srv.fetch(rid, r.size), oracle.gather(host_table, r, rb)), (round_, rid)
+    srv.close()
+
+
 def test_serve_submit_validates_arrays(trove, cuda):
     n_rows, rb = 1 << 10, 16
     _, table = _table(n_rows, rb, 23)
