@@ -1015,14 +1015,22 @@ def main():
         if c and nt:
             write_ceiling[str(rb)] = {"cached": c["gbs"], "nontemporal": nt["gbs"], "best": max(c["gbs"], nt["gbs"])}
     scatter_vs_ceiling = {}
+    memcpy_gbs = naive.get("memcpy_ceiling", {}).get("gbs")
     for k in ks:
         rb = 4 * k
         leg = naive.get(f"scatter_tile_k{k}")
         if str(rb) in write_ceiling and leg:
+            ceiling = write_ceiling[str(rb)]["best"]
             row_gbs = rb * n / (leg["avg_ms"] * 1e-3) / 1e9
-            scatter_vs_ceiling[str(rb)] = {"kernel": f"scatter_tile_k{k}", "row_write_gbs": round(row_gbs, 1),
-                                           "ceiling_gbs": write_ceiling[str(rb)]["best"],
-                                           "ceiling_frac": round(row_gbs / write_ceiling[str(rb)]["best"], 3)}
+            rec = {"kernel": f"scatter_tile_k{k}", "row_write_gbs": round(row_gbs, 1), "ceiling_gbs": ceiling,
+                   "ceiling_frac": round(row_gbs / ceiling, 3)}
+            if memcpy_gbs:
+                # The scatter also streams its source rows and indices: bound =
+                # the row writes at the write ceiling + S + 4 streamed bytes per
+                # row at the copy rate.
+                bound_ms = (rb * n / (ceiling * 1e9) + (rb + 4) * n / (memcpy_gbs * 1e9)) * 1e3
+                rec.update(bound_ms=round(bound_ms, 3), bound_frac=round(bound_ms / leg["avg_ms"], 3))
+            scatter_vs_ceiling[str(rb)] = rec
 
     # Contiguous copy, small: 20-byte structs, N = 2^20 (21 MB: resident in
     # the 256 MB Infinity Cache, so this is a warm-cache number), checked

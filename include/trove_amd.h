@@ -106,7 +106,7 @@ int trove_gather(const void* src, const int32_t* idx, void* dst, int64_t n, int 
  * DIRECT: tests/benchmark.cu:121-128; TILE: coalesced loads + tile-indexed
  * stores with the transposition elided; LDS: dwordx4 loads staged through
  * LDS to one struct per lane, then tile-indexed stores. AUTO: DIRECT for 4
- * bytes, LDS for 8 and 16, else TILE. tile as for trove_gather. */
+ * bytes, LDS for 8 and 16, SHFL for 64, else TILE. tile as for trove_gather. */
 int trove_scatter(const void* src, const int32_t* idx, void* dst, int64_t n, int struct_bytes, int variant, int tile,
                   void* stream);
 

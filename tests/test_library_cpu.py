@@ -89,7 +89,7 @@ def test_auto_variant_table():
     assert [T.auto_variant("gather", b) for b in (4, 16, 20, 48, 64, 192, 224, 384, 512, 2)] == \
         ["tile", "lds", "tile", "lds", "lds", "lds", "shfl", "lds", "tile", "direct"]
     assert [T.auto_variant("scatter", b) for b in (4, 8, 16, 20, 64, 6)] == \
-        ["direct", "lds", "lds", "tile", "tile", "direct"]
+        ["direct", "lds", "lds", "tile", "shfl", "direct"]
     assert T.lib().trove_auto_variant(3, 20) == T.ERR_INVALID
     assert T.lib().trove_auto_variant(0, 68) == T.ERR_SIZE
 

@@ -27,10 +27,11 @@ constexpr int kMaxLdsStructBytes = 512;
 // instead of narrower per-lane words) and the 16-byte-word structs of 32 and
 // 96-160 bytes (no two-rotation schedule); gathers: LDS-staged at 16, 48,
 // 64, 192 and 384 bytes (+6-9% over the tile path at 384), coalesced_ptr at
-// 224 bytes (+6-7%); scatters: LDS-staged at 8 and 16 bytes. 4-byte
-// scatters keep the naive kernel (0.93 TB/s; the streaming one-word tile
-// scatter reaches 0.84). Everything else takes the transposition (copy) or
-// the tile path (gather, scatter).
+// 224 bytes (+6-7%); scatters: LDS-staged at 8 and 16 bytes, coalesced_ptr
+// at 64 bytes (+4%: 3.69 against 3.54 TB/s, interleaved). 4-byte scatters
+// keep the naive kernel (0.93 TB/s; the streaming one-word tile scatter
+// reaches 0.84). Everything else takes the transposition (copy) or the tile
+// path (gather, scatter).
 constexpr int auto_copy_variant(int b) {
     return (b == 4 || b == 8 || b == 12 || b == 16 || b == 32 || b == 96 || b == 128 || b == 160)
                ? TROVE_VARIANT_LDS
@@ -42,7 +43,10 @@ constexpr int auto_gather_variant(int b) {
                                                                      : TROVE_VARIANT_TILE;
 }
 constexpr int auto_scatter_variant(int b) {
-    return b == 4 ? TROVE_VARIANT_DIRECT : (b == 8 || b == 16) ? TROVE_VARIANT_LDS : TROVE_VARIANT_TILE;
+    return b == 4 ? TROVE_VARIANT_DIRECT
+           : (b == 8 || b == 16) ? TROVE_VARIANT_LDS
+           : b == 64 ? TROVE_VARIANT_SHFL
+                     : TROVE_VARIANT_TILE;
 }
 // trove_scatter_permutation: struct sizes where inverting the permutation
 // (one random 4-byte write per row) and gathering through the inverse beat

@@ -92,7 +92,7 @@ enum req_state { QUEUED = 0, ACTIVE = 1, DONE = 2 };
 
 struct request {
     uint64_t id = 0;
-    const int32_t* idx = nullptr;  // caller's device indices, or host_idx.data()
+    const int32_t* idx = nullptr;  // caller's device indices (host indices: host_idx)
     std::vector<int32_t> host_idx;  // copy of host indices, taken at submission
     bool idx_on_device = false;
     bool live = false;       // the slot holds a request (request_table)
@@ -266,8 +266,8 @@ static int admit(trove_serve* s) {
             r.src_idx = r.idx;
         } else {
             int32_t* dst = s->idx_arena + row0;
-            const int rc = hip_status(
-                hipMemcpyAsync(dst, r.idx, sizeof(int32_t) * r.n, hipMemcpyHostToDevice, s->copy_stream));
+            const int rc = hip_status(hipMemcpyAsync(dst, r.host_idx.data(), sizeof(int32_t) * r.n,
+                                                     hipMemcpyHostToDevice, s->copy_stream));
             if (rc != TROVE_OK) return rc;
             r.src_idx = dst;
             uploaded = true;
@@ -430,7 +430,6 @@ int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on
         // Copied right after the check: later changes to the caller's array
         // (or its release) cannot reach the gather.
         r.host_idx.assign(idx, idx + n);
-        r.idx = r.host_idx.data();
     }
     r.n = n;
     r.rows = n > 0 ? n : 1;
