@@ -392,6 +392,69 @@ __device__ __forceinline__ typename enable_if<use_direct<T, Tile::size()>::value
 
 }  // namespace detail
 
+namespace detail {
+
+// *dst = *src for the structs of one converged tile, without the struct ever
+// sitting in one lane: the tile's source words are loaded in the column-side
+// layout (word j of rank id = word (id + j*n) mod m of the struct of rank
+// (id + j*n) / m) and stored in the same layout to the destination structs
+// of the same ranks, so neither R2C nor C2R runs. Every load of the tile is
+// issued before its first store, as when the struct passes through
+// registers, so overlapping source and destination read the old values.
+template <typename LoadP, typename StoreP, typename T, typename Tile>
+__device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value>::type copy_dispatch(const T* src, T* dst,
+                                                                                    const Tile& tile) {
+    constexpr int n = Tile::size();
+    typedef word_layout<T> L;
+    const int id = tile.id();
+    const int first = tile.first_lane();
+    uint32_t v[L::m * L::W];
+    const T* s0 = tile_first_pointer<n>(src, first);
+    const tile_offset os = offset_from(src, s0);
+    if (tile_all<n>(os.exact && os.delta == id)) {
+        tile_load_words<n, T, LoadP>(s0, id, v);
+    } else {
+        tile_gather_words<n, LoadP>(src, s0, os, id, first, v);
+    }
+    T* d0 = tile_first_pointer<n>(dst, first);
+    const tile_offset od = offset_from(dst, d0);
+    if (tile_all<n>(od.exact && od.delta == id)) {
+        tile_store_words<n, T, StoreP>(d0, id, v);
+    } else {
+        tile_scatter_words<n, StoreP>(dst, d0, od, id, first, v);
+    }
+}
+
+template <typename LoadP, typename StoreP, typename T, typename Tile>
+__device__ __forceinline__ typename enable_if<use_direct<T, Tile::size()>::value>::type copy_dispatch(const T* src, T* dst,
+                                                                                      const Tile&) {
+    divergent_store<StoreP>(divergent_load<LoadP>(src), dst);
+}
+
+// *dst = *src with the widest converged tile around the lane (the tile
+// cascade of load / store, decided once for both sides): what assigning one
+// coalesced_ref to another does.
+template <typename LoadP, typename StoreP, typename T>
+__device__ __forceinline__ void copy_cascade(const T* src, T* dst) {
+    if (warp_converged<WARP_SIZE>()) {
+        copy_dispatch<LoadP, StoreP>(src, dst, thread_tile<WARP_SIZE>());
+    } else if (warp_converged<WARP_SIZE / 2>()) {
+        copy_dispatch<LoadP, StoreP>(src, dst, thread_tile<WARP_SIZE / 2>());
+    } else if (warp_converged<WARP_SIZE / 4>()) {
+        copy_dispatch<LoadP, StoreP>(src, dst, thread_tile<WARP_SIZE / 4>());
+    } else if (warp_converged<WARP_SIZE / 8>()) {
+        copy_dispatch<LoadP, StoreP>(src, dst, thread_tile<WARP_SIZE / 8>());
+    } else if (warp_converged<WARP_SIZE / 16>()) {
+        copy_dispatch<LoadP, StoreP>(src, dst, thread_tile<WARP_SIZE / 16>());
+    } else if (warp_converged<WARP_SIZE / 32>()) {
+        copy_dispatch<LoadP, StoreP>(src, dst, thread_tile<WARP_SIZE / 32>());
+    } else {
+        divergent_store<StoreP>(divergent_load<LoadP>(src), dst);
+    }
+}
+
+}  // namespace detail
+
 // Load *src cooperatively with the widest converged tile around the lane.
 // Policy (an addition, defaulted to the reference's behaviour) is the cache
 // policy of the global accesses: trove::cached or trove::streaming.

@@ -3,9 +3,11 @@
 //
 // Public interface of trove/ptr.h:34-90. Reading a coalesced_ref calls
 // trove::load, assigning a T calls trove::store. Assigning one coalesced_ref
-// to another (dst[i] = src[j]) loads through trove::load and stores through
-// trove::store; the reference's version of that operator omits the tile
-// argument of its dispatch calls and cannot be instantiated (ptr.h:47-50).
+// to another (dst[i] = src[j]) moves the structs with the same tile cascade
+// but without the two transpositions a load followed by a store would run
+// (detail::copy_cascade); the reference's version of that operator omits the
+// tile argument of its dispatch calls and cannot be instantiated
+// (ptr.h:47-50).
 #pragma once
 #include <trove/aos.h>
 
@@ -25,16 +27,14 @@ struct coalesced_ref {
     }
 
     __device__ __forceinline__ coalesced_ref& operator=(const coalesced_ref& other) {
-        const T data = trove::load<T, Policy>(other.m_ptr);
-        trove::store<T, Policy>(data, m_ptr);
+        trove::detail::copy_cascade<Policy, Policy>(other.m_ptr, m_ptr);
         return *this;
     }
 
     // From a reference of the other policy: each side keeps its own.
     template <typename P, typename = typename std::enable_if<!std::is_same<P, Policy>::value>::type>
     __device__ __forceinline__ coalesced_ref& operator=(const coalesced_ref<T, P>& other) {
-        const T data = trove::load<T, P>(other.m_ptr);
-        trove::store<T, Policy>(data, m_ptr);
+        trove::detail::copy_cascade<P, Policy>(other.m_ptr, m_ptr);
         return *this;
     }
 };
