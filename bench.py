@@ -370,10 +370,19 @@ def bench_served(args, torch, T):
     # saturation and up to 1.5x the p99 at 95% load).
     slice_indices = 1 << 18
 
+    marker = torch.zeros(8, dtype=torch.int32, device=table.device)
+
+    def mark_run():
+        # A one-block marker copy before each run (outside its timing), so
+        # that a kernel trace of this config splits into its runs.
+        T.memcpy_ceiling(marker[:4], marker[4:], 16)
+        torch.cuda.synchronize()
+
     def run(arrivals, slice_indices=slice_indices):
         k = len(arrivals)
         srv = T.Server(table, n_rows, rb, budget_bytes=budget, max_slots=args.serve_slots,
                        slice_indices=slice_indices, variant=args.serve_variant)
+        mark_run()
         t_start = time.perf_counter()
         lat = srv.replay(arrivals, ptrs[:k], sizes[:k], on_device=True, release_on_done=True)
         wall = time.perf_counter() - t_start
@@ -422,6 +431,7 @@ def bench_served(args, torch, T):
     # Bit-exactness: a few requests alone vs the oracle.
     import oracle
 
+    mark_run()
     host_table = table.cpu().numpy().view(np.uint8)
     ok = True
     for i in range(0, R, max(1, R // 6))[:6]:
@@ -526,16 +536,28 @@ def trace_from(directory: str, timeline=None, n_log2: int = 26) -> int:
 
     rows = _trace_rows(directory)
     if timeline:
-        sel = [r for r in rows if _short_kernel(r[0]).startswith(timeline)]
-        if not sel:
+        # Runs are split at marker dispatches (a one-block k_memcpy16, grid
+        # 256) where the program placed them, else at pauses over 2 ms.
+        def is_marker(r):
+            return "k_memcpy16" in r[0] and r[1] == 256
+
+        markers = any(is_marker(r) for r in rows)
+        sel = [r for r in rows if _short_kernel(r[0]).startswith(timeline) or (markers and is_marker(r))]
+        if not any(not is_marker(r) for r in sel):
             raise SystemExit(f"no dispatch of a kernel starting with {timeline!r}")
-        run, prev_end = 0, None
+        run, prev_end, started = 0, None, False
         print("| run | step | kernel | grid (threads) | duration us | gap before us |")
         print("|---|---|---|---|---|---|")
         step = 0
-        for name, grid, t0, t1 in sel:
+        for r in sel:
+            name, grid, t0, t1 = r
+            if is_marker(r):
+                if started:
+                    run, step, prev_end = run + 1, 0, None
+                continue
+            started = True
             gap = None if prev_end is None else (t0 - prev_end) / 1e3
-            if gap is not None and gap > 2000:
+            if not markers and gap is not None and gap > 2000:
                 run, step, gap = run + 1, 0, None
             print(f"| {run} | {step} | `{_short_kernel(name).split('<')[0]}` | {grid} | {(t1 - t0) / 1e3:.1f} | "
                   f"{'' if gap is None else f'{gap:.1f}'} |")
