@@ -1050,8 +1050,8 @@ def main():
                 # The scatter also streams its source rows and indices: bound =
                 # the row writes at the write ceiling + S + 4 streamed bytes per
                 # row at the copy rate.
-                bound_ms = (rb * n / (ceiling * 1e9) + (rb + 4) * n / (memcpy_gbs * 1e9)) * 1e3
-                rec.update(bound_ms=round(bound_ms, 3), bound_frac=round(bound_ms / leg["avg_ms"], 3))
+                sc_ms = (rb * n / (ceiling * 1e9) + (rb + 4) * n / (memcpy_gbs * 1e9)) * 1e3
+                rec.update(bound_ms=round(sc_ms, 3), bound_frac=round(sc_ms / leg["avg_ms"], 3))
             scatter_vs_ceiling[str(rb)] = rec
 
     # Contiguous copy, small: 20-byte structs, N = 2^20 (21 MB: resident in
