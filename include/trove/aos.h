@@ -398,8 +398,8 @@ namespace detail {
 // sitting in one lane: the tile's source words are loaded in the column-side
 // layout (word j of rank id = word (id + j*n) mod m of the struct of rank
 // (id + j*n) / m) and stored in the same layout to the destination structs
-// of the same ranks, so neither R2C nor C2R runs. Every load of the tile is
-// issued before its first store, as when the struct passes through
+// of the same ranks, so neither R2C nor C2R runs. Every load of the tile
+// completes before its first store, as when the struct passes through
 // registers, so overlapping source and destination read the old values.
 template <typename LoadP, typename StoreP, typename T, typename Tile>
 __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value>::type copy_dispatch(const T* src, T* dst,
@@ -416,6 +416,9 @@ __device__ __forceinline__ typename enable_if<use_shfl<T, Tile::size()>::value>:
     } else {
         tile_gather_words<n, LoadP>(src, s0, os, id, first, v);
     }
+    // A lane stores words other lanes of the tile loaded from the same
+    // addresses when source and destination overlap: wait for every load.
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     T* d0 = tile_first_pointer<n>(dst, first);
     const tile_offset od = offset_from(dst, d0);
     if (tile_all<n>(od.exact && od.delta == id)) {

@@ -132,7 +132,8 @@ __global__ __launch_bounds__(64) void k_cascade(uint64_t mask, int scatter, cons
 // The same through the streaming cache policy: mode 0 calls
 // trove::load / trove::store<T, streaming>, mode 1 assigns between two
 // streaming coalesced_ptrs, mode 2 assigns a cached coalesced_ptr's element
-// to a streaming one (the cross-policy ref = ref).
+// to a streaming one (the cross-policy ref = ref), mode 3 assigns between two
+// cached coalesced_ptrs.
 template <int S>
 __global__ __launch_bounds__(64) void k_cascade_policy(uint64_t mask, int scatter, int mode,
                                                        const trove::array<uint32_t, S / 4>* src, const int* perm,
@@ -148,11 +149,30 @@ __global__ __launch_bounds__(64) void k_cascade_policy(uint64_t mask, int scatte
             trove::coalesced_ptr<T, NT> s(const_cast<T*>(src));
             trove::coalesced_ptr<T, NT> d(dst);
             d[to] = s[from];
-        } else {
+        } else if (mode == 2) {
             trove::coalesced_ptr<T> s(const_cast<T*>(src));
             trove::coalesced_ptr<T, NT> d(dst);
             d[to] = s[from];
+        } else {
+            trove::coalesced_ptr<T> s(const_cast<T*>(src));
+            trove::coalesced_ptr<T> d(dst);
+            d[to] = s[from];
         }
+    }
+}
+
+// ref = ref within one array: the wavefront permutes the 64 structs of buf
+// in place (scatter: buf[perm[lane]] = buf[lane]; gather: buf[lane] =
+// buf[perm[lane]]); every lane must read the structs as they were before.
+template <int S>
+__global__ __launch_bounds__(64) void k_inplace(int scatter, trove::array<uint32_t, S / 4>* buf, const int* perm) {
+    typedef trove::array<uint32_t, S / 4> T;
+    const int lane = threadIdx.x;
+    trove::coalesced_ptr<T> p(buf);
+    if (scatter) {
+        p[perm[lane]] = p[lane];
+    } else {
+        p[lane] = p[perm[lane]];
     }
 }
 
@@ -331,6 +351,18 @@ int devtest_cascade_policy(int s, uint64_t mask, int scatter, int mode, const vo
     }
     CP(8) CP(20) CP(32) CP(60) CP(64) CP(96) CP(256)
 #undef CP
+    return -1;
+}
+
+int devtest_inplace(int s, int scatter, void* buf, const int* perm) {
+#define IP(S)                                                                                        \
+    if (s == S) {                                                                                    \
+        hipLaunchKernelGGL((k_inplace<S>), dim3(1), dim3(64), 0, 0, scatter,                         \
+                           (trove::array<uint32_t, S / 4>*)buf, perm);                                \
+        return hipDeviceSynchronize() == hipSuccess ? 0 : -2;                                        \
+    }
+    IP(8) IP(20) IP(32) IP(60) IP(64) IP(96) IP(256)
+#undef IP
     return -1;
 }
 

@@ -46,6 +46,8 @@ def dt(cuda):
     L.devtest_cascade.argtypes = [i, u64, i, vp, vp, vp]
     L.devtest_cascade_policy.argtypes = [i, u64, i, i, vp, vp, vp]
     L.devtest_cascade_policy.restype = i
+    L.devtest_inplace.argtypes = [i, i, vp, vp]
+    L.devtest_inplace.restype = i
     L.devtest_shared_gather.argtypes = [i, u64, vp, vp, vp]
     L.devtest_shared_gather.restype = i
     L.devtest_mixed.argtypes = [i, u64, vp, vp, vp, vp]
@@ -186,16 +188,38 @@ def test_cascade_under_masks(dt, s):
 
 
 @pytest.mark.parametrize("s", [8, 20, 32, 60, 64, 96, 256])
+def test_ref_assignment_in_place(dt, s):
+    # coalesced_ref = coalesced_ref inside one array: a wavefront permutes 64
+    # structs in place; every lane reads the structs as they were before.
+    rng = np.random.default_rng(950 + s)
+    k = s // 4
+    for trial in range(4):
+        for scatter in (0, 1):
+            perm = rng.permutation(64).astype(np.int32)
+            old = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32).reshape(64, k)
+            d_buf = torch.from_numpy(old.reshape(-1).view(np.int32).copy()).cuda()
+            assert dt.devtest_inplace(s, scatter, d_buf.data_ptr(), torch.from_numpy(perm).cuda().data_ptr()) == 0
+            got = d_buf.cpu().numpy().view(np.uint32).reshape(64, k)
+            want = np.empty_like(old)
+            if scatter:
+                want[perm] = old
+            else:
+                want[:] = old[perm]
+            assert np.array_equal(got, want), (s, trial, scatter)
+
+
+@pytest.mark.parametrize("s", [8, 20, 32, 60, 64, 96, 256])
 def test_streaming_policy_cascade(dt, s):
     # The cascade with nontemporal accesses (trove::streaming) moves the same
     # structs as the cached one: trove::load / store<T, streaming>, two
-    # streaming coalesced_ptrs, and a cached source assigned to a streaming
-    # destination; gathers and scatters under arbitrary lane masks.
+    # streaming coalesced_ptrs, a cached source assigned to a streaming
+    # destination, and two cached coalesced_ptrs; gathers and scatters under
+    # arbitrary lane masks.
     rng = np.random.default_rng(900 + s)
     k = s // 4
     for mask in _masks()[:10]:
         for scatter in (0, 1):
-            for mode in (0, 1, 2):
+            for mode in (0, 1, 2, 3):
                 perm = rng.permutation(64).astype(np.int32)
                 src = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
                 init = rng.integers(0, 2**32, size=64 * k, dtype=np.uint64).astype(np.uint32)
