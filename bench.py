@@ -378,13 +378,18 @@ def bench_served(args, torch, T):
         T.memcpy_ceiling(marker[:4], marker[4:], 16)
         torch.cuda.synchronize()
 
-    def run(arrivals, slice_indices=slice_indices):
+    # The same requests' indices in host memory (each submission range-checks
+    # and copies its indices; the server uploads them before the step).
+    host_ptrs = [host_idx.ctypes.data + 4 * int(o) for o in offs]
+
+    def run(arrivals, slice_indices=slice_indices, on_device=True):
         k = len(arrivals)
         srv = T.Server(table, n_rows, rb, budget_bytes=budget, max_slots=args.serve_slots,
                        slice_indices=slice_indices, variant=args.serve_variant)
         mark_run()
         t_start = time.perf_counter()
-        lat = srv.replay(arrivals, ptrs[:k], sizes[:k], on_device=True, release_on_done=True)
+        lat = srv.replay(arrivals, (ptrs if on_device else host_ptrs)[:k], sizes[:k], on_device=on_device,
+                         release_on_done=True)
         wall = time.perf_counter() - t_start
         st = srv.stats()
         srv.close()
@@ -410,7 +415,22 @@ def bench_served(args, torch, T):
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
         "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"],
         "requests_spanning_steps": int((sizes > (1 << 16)).sum())}
+    # Indices handed over in host memory: the rate includes the host-side
+    # check and copy and the upload of 4 bytes per index over PCIe.
+    lat, wall, st = run(np.zeros(R), on_device=False)
+    host_rate = R / wall
+    out["saturation_host_indices"] = {
+        "req_per_s": round(host_rate, 1), "gbs": round(total_bytes / wall / 1e9, 1),
+        "index_upload_gbs": round(4 * int(sizes.sum()) / wall / 1e9, 2),
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"]}
     rng = np.random.default_rng(args.seed + 13)
+    arrivals = np.cumsum(rng.exponential(1.0 / (0.5 * host_rate), size=R))
+    lat, wall, st = run(arrivals, on_device=False)
+    out["poisson_host_indices_50pct"] = {
+        "rate_req_per_s": round(0.5 * host_rate, 1), "gbs": round(total_bytes / wall / 1e9, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "steps": st["steps"]}
     for frac in (0.25, 0.5, 0.8, 0.95):
         rate = frac * sat_rate
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=R))

@@ -32,6 +32,7 @@
 #include <chrono>
 #include <deque>
 #include <map>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -93,7 +94,7 @@ enum req_state { QUEUED = 0, ACTIVE = 1, DONE = 2 };
 struct request {
     uint64_t id = 0;
     const int32_t* idx = nullptr;  // caller's device indices (host indices: host_idx)
-    std::vector<int32_t> host_idx;  // copy of host indices, taken at submission
+    std::unique_ptr<int32_t[]> host_idx;  // copy of host indices, taken at submission
     bool idx_on_device = false;
     bool live = false;       // the slot holds a request (request_table)
     bool bad_index = false;  // a step flagged an index outside the table
@@ -143,7 +144,7 @@ class request_table {
     }
     void erase(request& r) {
         r.live = false;
-        std::vector<int32_t>().swap(r.host_idx);
+        r.host_idx.reset();
         free_.push_back((uint32_t)(r.id & 0xffffffffu));
     }
     // Room for n more requests without moving the slots.
@@ -239,7 +240,7 @@ static int retire(trove_serve* s) {
             request& r = s->reqs[id];
             r.state = DONE;
             r.t_done = t;
-            std::vector<int32_t>().swap(r.host_idx);  // uploaded and gathered: no longer needed
+            r.host_idx.reset();  // uploaded and gathered: no longer needed
             s->completed++;
             s->indices_done += r.n;
             if (s->replaying) s->newly_done.push_back(id);
@@ -266,7 +267,7 @@ static int admit(trove_serve* s) {
             r.src_idx = r.idx;
         } else {
             int32_t* dst = s->idx_arena + row0;
-            const int rc = hip_status(hipMemcpyAsync(dst, r.host_idx.data(), sizeof(int32_t) * r.n,
+            const int rc = hip_status(hipMemcpyAsync(dst, r.host_idx.get(), sizeof(int32_t) * r.n,
                                                      hipMemcpyHostToDevice, s->copy_stream));
             if (rc != TROVE_OK) return rc;
             r.src_idx = dst;
@@ -360,6 +361,20 @@ static int serve_step(trove_serve* s) {
     return rc;
 }
 
+// dst[i] = src[i] for i < n; false when some index lies outside [0, n_rows).
+// One pass with no early exit, so the compiler vectorises it (an unsigned
+// compare also catches negative indices); the caller discards dst on false.
+static bool copy_checked(int32_t* __restrict__ dst, const int32_t* __restrict__ src, int64_t n, int64_t n_rows) {
+    const uint32_t lim = (uint32_t)n_rows;  // n_rows <= INT32_MAX (trove_serve_create)
+    uint32_t bad = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t v = src[i];
+        dst[i] = v;
+        bad |= (uint32_t)((uint32_t)v >= lim);
+    }
+    return bad == 0;
+}
+
 // Zero-length requests are done at submission.
 static void finish_empty(trove_serve* s, request& r) {
     r.state = DONE;
@@ -396,6 +411,7 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
     if (rc == TROVE_OK) rc = hip_status(hipEventCreateWithFlags(&s->upload_done, hipEventDisableTiming));
     if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->out_arena, (size_t)(s->capacity_rows * row_bytes)));
     if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->idx_arena, (size_t)(s->capacity_rows * sizeof(int32_t))));
+
     for (int k = 0; k < kInflight && rc == TROVE_OK; ++k) {
         rc = hip_status(hipEventCreateWithFlags(&s->steps[k].done, hipEventDisableTiming));
         if (rc == TROVE_OK) rc = hip_status(hipHostMalloc((void**)&s->h_jobs[k], sizeof(job_desc) * s->max_jobs));
@@ -416,9 +432,15 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
 int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on_device, uint64_t* request_id) {
     if (!s || !request_id || n < 0 || (n > 0 && !idx)) return TROVE_ERR_INVALID;
     if (n > s->capacity_rows) return TROVE_ERR_BUDGET;
+    std::unique_ptr<int32_t[]> copy;
     if (!idx_on_device) {
-        for (int64_t i = 0; i < n; ++i)
-            if (idx[i] < 0 || idx[i] >= s->n_rows) return TROVE_ERR_RANGE;
+        // Checked and copied in one pass: later changes to the caller's array
+        // (or its release) cannot reach the gather.
+        if (n > 0) {
+            copy.reset(new (std::nothrow) int32_t[n]);
+            if (!copy) return TROVE_ERR_NOMEM;
+            if (!copy_checked(copy.get(), idx, n, s->n_rows)) return TROVE_ERR_RANGE;
+        }
     } else if (!aligned(idx, 4)) {
         return TROVE_ERR_ALIGN;
     }
@@ -426,10 +448,8 @@ int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on
     r.idx_on_device = idx_on_device != 0;
     if (r.idx_on_device) {
         r.idx = idx;
-    } else if (n > 0) {
-        // Copied right after the check: later changes to the caller's array
-        // (or its release) cannot reach the gather.
-        r.host_idx.assign(idx, idx + n);
+    } else {
+        r.host_idx = std::move(copy);
     }
     r.n = n;
     r.rows = n > 0 ? n : 1;
