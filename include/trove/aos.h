@@ -390,10 +390,6 @@ __device__ __forceinline__ typename enable_if<use_direct<T, Tile::size()>::value
     divergent_store<Policy>(data, dest);
 }
 
-}  // namespace detail
-
-namespace detail {
-
 // *dst = *src for the structs of one converged tile, without the struct ever
 // sitting in one lane: the tile's source words are loaded in the column-side
 // layout (word j of rank id = word (id + j*n) mod m of the struct of rank

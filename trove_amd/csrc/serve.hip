@@ -411,7 +411,6 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
     if (rc == TROVE_OK) rc = hip_status(hipEventCreateWithFlags(&s->upload_done, hipEventDisableTiming));
     if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->out_arena, (size_t)(s->capacity_rows * row_bytes)));
     if (rc == TROVE_OK) rc = hip_status(hipMalloc(&s->idx_arena, (size_t)(s->capacity_rows * sizeof(int32_t))));
-
     for (int k = 0; k < kInflight && rc == TROVE_OK; ++k) {
         rc = hip_status(hipEventCreateWithFlags(&s->steps[k].done, hipEventDisableTiming));
         if (rc == TROVE_OK) rc = hip_status(hipHostMalloc((void**)&s->h_jobs[k], sizeof(job_desc) * s->max_jobs));
