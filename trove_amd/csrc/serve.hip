@@ -102,6 +102,7 @@ struct request {
     int64_t rows = 0;       // arena rows reserved (>= 1)
     int64_t row0 = -1;      // arena offset
     int64_t scheduled = 0;  // indices handed to steps so far
+    int64_t stage0 = -1;    // host indices uploaded at submission: their offset in the staging pool
     int state = QUEUED;
     double t_submit = 0, t_done = 0;
     const int32_t* src_idx = nullptr;  // indices the kernel reads (caller's or arena copy)
@@ -189,6 +190,13 @@ struct trove_serve {
     int32_t* idx_arena = nullptr;
     range_allocator* arena = nullptr;
     hipEvent_t upload_done = nullptr;  // last host-index upload pass
+    // Device staging pool for host indices uploaded at submission (allocated
+    // at the first host-index submission, capacity_rows entries); a request
+    // that finds it full keeps a host copy and uploads it at admission.
+    int32_t* dstage = nullptr;
+    range_allocator* dstage_alloc = nullptr;
+    bool dstage_tried = false;
+    bool upload_pending = false;  // submission uploads not yet ordered before a step
     std::deque<uint64_t> queue;   // QUEUED, FIFO
     std::vector<uint64_t> active;  // ACTIVE, admission order
     std::vector<uint64_t> still_active;  // launch_step's next `active` (kept for its capacity)
@@ -220,6 +228,8 @@ static void free_serve(trove_serve* s) {
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->out_arena) (void)hipFree(s->out_arena);
     if (s->idx_arena) (void)hipFree(s->idx_arena);
+    if (s->dstage) (void)hipFree(s->dstage);
+    delete s->dstage_alloc;
     delete s->arena;
     delete s;
 }
@@ -241,6 +251,10 @@ static int retire(trove_serve* s) {
             r.state = DONE;
             r.t_done = t;
             r.host_idx.reset();  // uploaded and gathered: no longer needed
+            if (r.stage0 >= 0) {
+                s->dstage_alloc->release(r.stage0, r.n);
+                r.stage0 = -1;
+            }
             s->completed++;
             s->indices_done += r.n;
             if (s->replaying) s->newly_done.push_back(id);
@@ -265,6 +279,8 @@ static int admit(trove_serve* s) {
         r.row0 = row0;
         if (r.idx_on_device) {
             r.src_idx = r.idx;
+        } else if (r.stage0 >= 0) {
+            r.src_idx = s->dstage + r.stage0;  // uploaded at submission
         } else {
             int32_t* dst = s->idx_arena + row0;
             const int rc = hip_status(hipMemcpyAsync(dst, r.host_idx.get(), sizeof(int32_t) * r.n,
@@ -277,7 +293,10 @@ static int admit(trove_serve* s) {
         s->active.push_back(r.id);
         s->queue.pop_front();
     }
-    if (uploaded) {
+    if (uploaded || s->upload_pending) {
+        // One event after every upload issued so far (at submission and here)
+        // orders them before the next step's kernel.
+        s->upload_pending = false;
         int rc = hip_status(hipEventRecord(s->upload_done, s->copy_stream));
         if (rc == TROVE_OK) rc = hip_status(hipStreamWaitEvent(s->stream, s->upload_done, 0));
         if (rc != TROVE_OK) return rc;
@@ -361,6 +380,34 @@ static int serve_step(trove_serve* s) {
     return rc;
 }
 
+// Whether every src[i], i < n, lies in [0, n_rows): one read-only pass with
+// no early exit, so the compiler vectorises it (an unsigned compare also
+// catches negative indices).
+static bool all_in_range(const int32_t* __restrict__ src, int64_t n, int64_t n_rows) {
+    const uint32_t lim = (uint32_t)n_rows;  // n_rows <= INT32_MAX (trove_serve_create)
+    uint32_t bad = 0;
+    for (int64_t i = 0; i < n; ++i) bad |= (uint32_t)((uint32_t)src[i] >= lim);
+    return bad == 0;
+}
+
+// The device staging pool, allocated at the first host-index submission
+// (servers fed device indices never hold it). Without it (allocation
+// refused) every host-index request takes the host-copy path.
+static void ensure_dstage(trove_serve* s) {
+    if (s->dstage_tried) return;
+    s->dstage_tried = true;
+    if (hipMalloc(&s->dstage, (size_t)s->capacity_rows * sizeof(int32_t)) != hipSuccess) {
+        s->dstage = nullptr;
+        (void)hipGetLastError();
+        return;
+    }
+    s->dstage_alloc = new (std::nothrow) range_allocator(s->capacity_rows);
+    if (!s->dstage_alloc) {
+        (void)hipFree(s->dstage);
+        s->dstage = nullptr;
+    }
+}
+
 // dst[i] = src[i] for i < n; false when some index lies outside [0, n_rows).
 // One pass with no early exit, so the compiler vectorises it (an unsigned
 // compare also catches negative indices); the caller discards dst on false.
@@ -432,13 +479,32 @@ int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on
     if (!s || !request_id || n < 0 || (n > 0 && !idx)) return TROVE_ERR_INVALID;
     if (n > s->capacity_rows) return TROVE_ERR_BUDGET;
     std::unique_ptr<int32_t[]> copy;
+    int64_t stage0 = -1;
     if (!idx_on_device) {
-        // Checked and copied in one pass: later changes to the caller's array
+        // Host indices are checked, then uploaded to the staging pool right
+        // away (an upload from pageable memory has read the caller's array
+        // when it returns), or, with the pool full, checked and copied to the
+        // host in one pass; either way later changes to the caller's array
         // (or its release) cannot reach the gather.
         if (n > 0) {
-            copy.reset(new (std::nothrow) int32_t[n]);
-            if (!copy) return TROVE_ERR_NOMEM;
-            if (!copy_checked(copy.get(), idx, n, s->n_rows)) return TROVE_ERR_RANGE;
+            ensure_dstage(s);
+            if (s->dstage_alloc && (stage0 = s->dstage_alloc->alloc(n)) >= 0) {
+                if (!all_in_range(idx, n, s->n_rows)) {
+                    s->dstage_alloc->release(stage0, n);
+                    return TROVE_ERR_RANGE;
+                }
+                const int rc = hip_status(hipMemcpyAsync(s->dstage + stage0, idx, sizeof(int32_t) * n,
+                                                         hipMemcpyHostToDevice, s->copy_stream));
+                if (rc != TROVE_OK) {
+                    s->dstage_alloc->release(stage0, n);
+                    return rc;
+                }
+                s->upload_pending = true;
+            } else {
+                copy.reset(new (std::nothrow) int32_t[n]);
+                if (!copy) return TROVE_ERR_NOMEM;
+                if (!copy_checked(copy.get(), idx, n, s->n_rows)) return TROVE_ERR_RANGE;
+            }
         }
     } else if (!aligned(idx, 4)) {
         return TROVE_ERR_ALIGN;
@@ -449,6 +515,7 @@ int trove_serve_submit(trove_serve* s, const int32_t* idx, int64_t n, int idx_on
         r.idx = idx;
     } else {
         r.host_idx = std::move(copy);
+        r.stage0 = stage0;
     }
     r.n = n;
     r.rows = n > 0 ? n : 1;
