@@ -94,19 +94,21 @@ TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
 # The translation units whose kernels profiles/traffic.json measures (copy,
 # gather, scatter, store / load, block interface, ceilings, permutation
 # inversion) and the headers they include; the batch engine and the server
-# (batch.hip, batch.h, serve.hip) have no leg in that table.
+# (batch.hip, batch.h, serve.hip) have no leg in that table, and the C ABI
+# header (include/trove_amd.h) holds no device code.
 TRAFFIC_SOURCES = ("common.h", "gather_body.h", "copy.hip", "gather.hip", "block.hip", "capi.hip")
 
 
 def kernel_sources_sha256() -> str:
     """Hash of the sources of the kernels profiles/traffic.json measures
-    (TRAFFIC_SOURCES under trove_amd/csrc, and include/): the stamp that ties
-    the table to the kernels it was measured on."""
+    (TRAFFIC_SOURCES under trove_amd/csrc, and the device headers under
+    include/trove): the stamp that ties the table to the kernels it was
+    measured on."""
     import hashlib
 
     h = hashlib.sha256()
     files = [os.path.join(ROOT, "trove_amd", "csrc", f) for f in TRAFFIC_SOURCES]
-    for d, _, names in os.walk(os.path.join(ROOT, "include")):
+    for d, _, names in os.walk(os.path.join(ROOT, "include", "trove")):
         files += [os.path.join(d, f) for f in names if f.endswith((".hip", ".h", ".cpp"))]
     for f in sorted(files, key=lambda x: os.path.relpath(x, ROOT)):
         h.update(os.path.relpath(f, ROOT).encode())
@@ -663,7 +665,7 @@ def traffic_from(fetch_dir: str, write_dir: str, legs_json: str, out_path: str =
     f_fac = each_way / (c[1] / c[0] * 1024)
     w_fac = each_way / (c[2] / c[0] * 1024)
     try:
-        paths = [f"trove_amd/csrc/{f}" for f in TRAFFIC_SOURCES] + ["include"]
+        paths = [f"trove_amd/csrc/{f}" for f in TRAFFIC_SOURCES] + ["include/trove"]
         commit = subprocess.run(["git", "-C", ROOT, "log", "-1", "--format=%h", "--", *paths],
                                 capture_output=True, text=True, timeout=20).stdout.strip() or None
     except (OSError, subprocess.SubprocessError):
