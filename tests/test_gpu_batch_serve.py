@@ -309,6 +309,40 @@ def test_serve_copies_host_indices_at_submit(trove, cuda):
     srv.close()
 
 
+def test_serve_host_indices_beyond_the_staging_pool(trove, cuda):
+    # Host indices upload at submission into a staging pool as large as the
+    # output budget's rows; submissions that find it full keep a host copy
+    # and upload at admission. Both kinds, mixed, gather the right rows, and
+    # the caller's arrays are overwritten right after each submission.
+    n_rows, rb = 1 << 12, 16
+    host_table, table = _table(n_rows, rb, 24)
+    srv = T.Server(table, n_rows, rb, budget_bytes=4096 * rb, max_slots=8, slice_indices=512)
+    rng = np.random.default_rng(24)
+    reqs = [rng.integers(0, n_rows, size=1500, dtype=np.int32) for _ in range(7)]  # 10,500 > 4,096 rows
+    wants = [oracle.gather(host_table, r, rb) for r in reqs]
+    pending = {}
+    for r, want in zip(reqs, wants):
+        pending[srv.submit(r, r.size, on_device=False)] = want
+        r[:] = 0  # the submission already holds its own indices
+    # The outputs of all requests exceed the budget: release each as it
+    # finishes so that the queue drains.
+    for _ in range(200000):
+        if not pending:
+            break
+        srv.step()
+        for rid in list(pending):
+            if srv.poll(rid)[0]:
+                assert np.array_equal(srv.fetch(rid), pending.pop(rid)), rid
+                srv.release(rid)
+    assert not pending
+    # Once released, the pool takes new submissions again.
+    r = rng.integers(0, n_rows, size=3000, dtype=np.int32)
+    rid = srv.submit(r, r.size, on_device=False)
+    srv.drain()
+    assert np.array_equal(srv.fetch(rid), oracle.gather(host_table, r, rb))
+    srv.close()
+
+
 def test_serve_request_ids_stay_unique_across_slot_reuse(trove, cuda):
     # A released request's table slot is reused by the next submission; the
     # old id must then name nothing, and every live id its own request.
