@@ -262,8 +262,9 @@ int trove_serve_create(const void* table, int64_t n_rows, int32_t row_bytes, int
                        int32_t max_slots, int64_t slice_indices, int32_t variant, void* stream,
                        trove_serve** out);
 /* Queues a request. Host indices are range-checked (TROVE_ERR_RANGE) and
- * uploaded to a device staging pool of the server (capacity as the output
- * budget's rows, allocated at the first host-index submission) before the
+ * uploaded to a device staging pool of the server (as many entries as the
+ * output budget has rows, at most 2^28; allocated at the first host-index
+ * submission) before the
  * call returns, or copied to host memory when that pool is full, so the
  * caller's array may change or be freed right after. Device indices are read in place and must stay valid
  * until the request is done; the step kernels check each of them against

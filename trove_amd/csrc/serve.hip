@@ -190,9 +190,9 @@ struct trove_serve {
     int32_t* idx_arena = nullptr;
     range_allocator* arena = nullptr;
     hipEvent_t upload_done = nullptr;  // last host-index upload pass
-    // Device staging pool for host indices uploaded at submission (allocated
-    // at the first host-index submission, capacity_rows entries); a request
-    // that finds it full keeps a host copy and uploads it at admission.
+    // Device staging pool for host indices uploaded at submission (see
+    // ensure_dstage); a request that finds it full keeps a host copy and
+    // uploads it at admission.
     int32_t* dstage = nullptr;
     range_allocator* dstage_alloc = nullptr;
     bool dstage_tried = false;
@@ -391,17 +391,20 @@ static bool all_in_range(const int32_t* __restrict__ src, int64_t n, int64_t n_r
 }
 
 // The device staging pool, allocated at the first host-index submission
-// (servers fed device indices never hold it). Without it (allocation
-// refused) every host-index request takes the host-copy path.
+// (servers fed device indices never hold it): as many entries as the output
+// budget has rows, at most kMaxStageInts. Without it (allocation refused)
+// every host-index request takes the host-copy path.
+constexpr int64_t kMaxStageInts = (int64_t)1 << 28;  // 1 GiB of device memory
 static void ensure_dstage(trove_serve* s) {
     if (s->dstage_tried) return;
     s->dstage_tried = true;
-    if (hipMalloc(&s->dstage, (size_t)s->capacity_rows * sizeof(int32_t)) != hipSuccess) {
+    const int64_t ints = s->capacity_rows < kMaxStageInts ? s->capacity_rows : kMaxStageInts;
+    if (hipMalloc(&s->dstage, (size_t)ints * sizeof(int32_t)) != hipSuccess) {
         s->dstage = nullptr;
         (void)hipGetLastError();
         return;
     }
-    s->dstage_alloc = new (std::nothrow) range_allocator(s->capacity_rows);
+    s->dstage_alloc = new (std::nothrow) range_allocator(ints);
     if (!s->dstage_alloc) {
         (void)hipFree(s->dstage);
         s->dstage = nullptr;
